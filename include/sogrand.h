@@ -1,0 +1,139 @@
+/*
+ * sogrand.h - C ABI of the MI355X (gfx950) SOGRAND decoder for the (16,8)^3
+ * cubic product code.
+ *
+ * All entry points are extern "C", take plain pointers and sizes, and return
+ * 0 on success or a negative SOGRAND_E* code (never exit()). Buffers may be
+ * host memory (pageable or pinned) or device memory; the library tells them
+ * apart with hipPointerGetAttributes. Results never depend on how codewords
+ * are batched or which backend decodes them.
+ *
+ * Data layout (the reference decoder's file formats):
+ *   LLR cube  : 4096 float64 per codeword, index a0 + 16*a1 + 256*a2
+ *               (SOGRAND_C/cubic_decoder_final.c:119, cubic_encoder.c:89-97)
+ *   info bits : 512 per codeword, bit b = a0 + 8*a1 + 64*a2 (a0,a1,a2 < 8),
+ *               packed MSB-first, 64 bytes per codeword (cubic_decoder_final.c:267-280)
+ */
+#ifndef SOGRAND_H
+#define SOGRAND_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SOGRAND_N 16
+#define SOGRAND_K 8
+#define SOGRAND_CUBE 4096     /* LLRs per codeword          */
+#define SOGRAND_INFO_BITS 512 /* info bits per codeword     */
+#define SOGRAND_INFO_BYTES 64 /* packed info bytes per cw   */
+#define SOGRAND_MAX_L 4       /* largest list size supported */
+
+/* error codes */
+#define SOGRAND_OK 0
+#define SOGRAND_EINVAL (-1)     /* bad argument                               */
+#define SOGRAND_ENOMEM (-2)     /* host or device allocation failed          */
+#define SOGRAND_EHIP (-3)       /* a HIP runtime call failed                   */
+#define SOGRAND_ENONFINITE (-4) /* an LLR was NaN or infinite                  */
+#define SOGRAND_ENODEV (-5)     /* no usable gfx950 device                     */
+#define SOGRAND_EIO (-6)        /* file open/read/write failed (CLI helpers)   */
+#define SOGRAND_EBUSY (-7)      /* serving queue full / admission refused      */
+
+/* precision */
+#define SOGRAND_FP64 0       /* bit-exact with SOGRAND_C                              */
+#define SOGRAND_FP32_MIXED 1 /* FP32 sort + pattern metrics, FP64 list normalisation */
+
+/* backend */
+#define SOGRAND_BACKEND_PLAIN 0      /* one workgroup per codeword, static lines */
+#define SOGRAND_BACKEND_PERSISTENT 1 /* persistent waves + device queue          */
+
+/* Decoder parameters. Defaults (sogrand_params_default) are the reference's
+ * constants: L=3, Imax=30, Tmax=UINT64_MAX, p_ET=1e-5, alpha=0.7
+ * (SOGRAND_C/cubic_decoder_final.c:94-102). */
+typedef struct sogrand_params {
+    uint32_t L;         /* list size, 1..SOGRAND_MAX_L                                  */
+    uint32_t Imax;      /* maximum product-code iterations (3 phases each), 1..1000     */
+    uint64_t Tmax;      /* guess budget per component decode, checked once per weight
+                           level as the reference does; 0 means 0x7fffffff              */
+    double p_ET;        /* list confidence threshold is 1 - p_ET                         */
+    double alpha;       /* extrinsic scaling                                             */
+    int32_t precision;  /* SOGRAND_FP64 or SOGRAND_FP32_MIXED                            */
+    int32_t backend;    /* SOGRAND_BACKEND_*                                             */
+    int32_t wave_slots; /* persistent: cube slots per wave (0 = tuned default)           */
+    int32_t waves_per_cu; /* persistent: resident waves per CU (0 = tuned default)       */
+} sogrand_params;
+
+/* Per-codeword statistics (the reference prints only their averages,
+ * cubic_decoder_final.c:299-304). */
+typedef struct sogrand_cw_stats {
+    uint32_t half_iters; /* phases run; the reference's n_iter = half_iters / 2 */
+    uint32_t converged;  /* 1 if the early-termination test passed              */
+    uint64_t NG;         /* guesses summed over all component decodes           */
+    uint64_t NG_p;       /* per-phase maximum guesses, summed over phases       */
+} sogrand_cw_stats;
+
+typedef struct sogrand_ctx sogrand_ctx;
+
+void sogrand_params_default(sogrand_params *p);
+const char *sogrand_strerror(int code);
+
+/* Create a decoder context on HIP device `device`. Uploads the error-pattern
+ * schedule tables. Replaces the reference's per-process set-up
+ * (cubic_decoder1.cu:950-985: getGH_sys_CRC, cudaMemcpyToSymbol). */
+int sogrand_ctx_create(sogrand_ctx **out, int device, const sogrand_params *params);
+int sogrand_ctx_set_params(sogrand_ctx *ctx, const sogrand_params *params);
+void sogrand_ctx_destroy(sogrand_ctx *ctx);
+
+/* Decode n_cw cubes. llr: 4096 f64 per codeword. info_bits: 64 B per codeword
+ * (required). soft_out (optional, may be NULL): final L_APP, 4096 f64 per
+ * codeword. stats (optional): one per codeword. hip_stream: hipStream_t or NULL
+ * for the context's stream. Synchronous: returns when results are written.
+ * Replaces the reference's decode loop, cubic_decoder_final.c:119-289, and the
+ * GPU launch + copy-back, cubic_decoder1.cu:1059-1110 (kernel boundary
+ * highly_optimized_sogrand_kernel, cubic_decoder1.cu:635-643). */
+int sogrand_decode_cubic(sogrand_ctx *ctx, const double *llr, size_t n_cw, uint8_t *info_bits,
+                         double *soft_out, sogrand_cw_stats *stats, void *hip_stream);
+
+/* Component SISO decode of n_lines independent 16-value lines on the GPU:
+ * L_APP, L_E (16 f64 per line) and N_guess per line. Replaces SOGRAND_bitSO
+ * (cubic_decoder_final.c:521) with L, Tmax, thres from the context.
+ * L_APP/L_E may be NULL. */
+int sogrand_bitso_lines(sogrand_ctx *ctx, const double *llr, size_t n_lines, double *L_APP,
+                        double *L_E, uint32_t *N_guess, void *hip_stream);
+
+/* Time (ms, HIP events on the launch stream) of the decode kernels of the last
+ * sogrand_decode_cubic call on this context, and how many launches it made. */
+int sogrand_last_kernel_ms(sogrand_ctx *ctx, double *ms, uint32_t *launches);
+
+/* ---- synthetic data, restating the reference's channel (channel_sim.c:34-39,61-89)
+ * with a seeded counter-based generator. Codeword c of a stream depends only on
+ * (seed, c): any chunking gives identical data; host and device agree bit for bit. */
+
+/* sigma of BPSK over AWGN at Eb/N0 (dB) for the rate-1/8 cube code
+ * (channel_sim.c:34-39: EsN0 = EbN0 + 10 log10(2R), sigma = 1/sqrt(10^(EsN0/10))). */
+double sogrand_sigma_for_ebn0(double ebn0_db);
+
+/* Info bits of codewords [cw_offset, cw_offset + n_cw): 64 B per codeword. */
+int sogrand_gen_info_bits(uint64_t seed, uint64_t cw_offset, size_t n_cw, uint8_t *info_bits);
+
+/* Systematic (16,8)^3 encoding of packed info bits into the reference
+ * encoder's output format: 4096 code bits per codeword, index a0+16*a1+256*a2,
+ * packed MSB-first, 512 B per codeword (cubic_encoder.c:23-100). Host only. */
+int sogrand_encode_cubes(const uint8_t *info_bits, size_t n_cw, uint8_t *code_bytes);
+
+/* LLRs (4096 f64 per codeword) for codewords [cw_offset, cw_offset + n_cw):
+ * info bits from (seed, c), encoded, BPSK 0 -> +1, AWGN with sigma[c - cw_offset]
+ * (n_sigma == n_cw) or sigma[0] (n_sigma == 1), LLR = 2y / sigma^2.
+ * sogrand_gen_llr runs on the host; sogrand_gen_llr_device writes device
+ * memory on the GPU (llr, sigma device or host pointers; hip_stream or NULL). */
+int sogrand_gen_llr(uint64_t seed, uint64_t cw_offset, size_t n_cw, const double *sigma,
+                    size_t n_sigma, double *llr);
+int sogrand_gen_llr_device(sogrand_ctx *ctx, uint64_t seed, uint64_t cw_offset, size_t n_cw,
+                           const double *sigma, size_t n_sigma, double *llr, void *hip_stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,261 @@
+"""sogrand_amd - SOGRAND decoder for the (16,8)^3 cubic product code on AMD MI355X (gfx950).
+
+Python view of the C ABI in include/sogrand.h (libsogrand.so, built in-tree by
+``python -m sogrand_amd.build``). The decoder itself is HIP code; this module
+only moves pointers. There is no CPU fallback: if the library is missing or no
+gfx950 device is present, every entry point raises.
+
+Mirrors the reference decoder's interface:
+  Decoder.decode(llr)      ~ ``cubic_decoder <llr_file> <out_file>`` (SOGRAND_C/cubic_decoder_final.c:62-314)
+  Decoder.bitso(lines)     ~ SOGRAND_bitSO (cubic_decoder_final.c:521)
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libsogrand.so")
+CLI_PATH = os.path.join(PKG, "cubic_decoder")
+
+OK = 0
+EINVAL = -1
+ENOMEM = -2
+EHIP = -3
+ENONFINITE = -4
+ENODEV = -5
+EIO = -6
+EBUSY = -7
+
+FP64 = 0
+FP32_MIXED = 1
+BACKEND_PLAIN = 0
+BACKEND_PERSISTENT = 1
+UINT64_MAX = (1 << 64) - 1
+
+CW_STATS_DTYPE = np.dtype(
+    [("half_iters", "<u4"), ("converged", "<u4"), ("NG", "<u8"), ("NG_p", "<u8")]
+)
+
+# The C-ABI symbols include/sogrand.h declares.
+EXPORTED = [
+    "sogrand_params_default", "sogrand_strerror", "sogrand_ctx_create", "sogrand_ctx_set_params",
+    "sogrand_ctx_destroy", "sogrand_decode_cubic", "sogrand_bitso_lines", "sogrand_last_kernel_ms",
+    "sogrand_sigma_for_ebn0", "sogrand_gen_info_bits", "sogrand_encode_cubes", "sogrand_gen_llr",
+    "sogrand_gen_llr_device",
+]
+
+
+class SograndError(RuntimeError):
+    def __init__(self, code: int, what: str = ""):
+        self.code = code
+        msg = _strerror(code)
+        super().__init__(f"{what}: {msg} ({code})" if what else f"{msg} ({code})")
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("L", C.c_uint32),
+        ("Imax", C.c_uint32),
+        ("Tmax", C.c_uint64),
+        ("p_ET", C.c_double),
+        ("alpha", C.c_double),
+        ("precision", C.c_int32),
+        ("backend", C.c_int32),
+        ("wave_slots", C.c_int32),
+        ("waves_per_cu", C.c_int32),
+    ]
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH) -> C.CDLL:
+    """Load libsogrand.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(
+            f"{path} is missing: build it with `python -m sogrand_amd.build` (hipcc, gfx950)"
+        )
+    lib = C.CDLL(path)
+    vp = C.c_void_p
+    lib.sogrand_params_default.argtypes = [C.POINTER(Params)]
+    lib.sogrand_params_default.restype = None
+    lib.sogrand_strerror.argtypes = [C.c_int]
+    lib.sogrand_strerror.restype = C.c_char_p
+    lib.sogrand_ctx_create.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(Params)]
+    lib.sogrand_ctx_set_params.argtypes = [vp, C.POINTER(Params)]
+    lib.sogrand_ctx_destroy.argtypes = [vp]
+    lib.sogrand_ctx_destroy.restype = None
+    lib.sogrand_decode_cubic.argtypes = [vp, vp, C.c_size_t, vp, vp, vp, vp]
+    lib.sogrand_bitso_lines.argtypes = [vp, vp, C.c_size_t, vp, vp, vp, vp]
+    lib.sogrand_last_kernel_ms.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_uint32)]
+    lib.sogrand_sigma_for_ebn0.argtypes = [C.c_double]
+    lib.sogrand_sigma_for_ebn0.restype = C.c_double
+    lib.sogrand_gen_info_bits.argtypes = [C.c_uint64, C.c_uint64, C.c_size_t, vp]
+    lib.sogrand_encode_cubes.argtypes = [vp, C.c_size_t, vp]
+    lib.sogrand_gen_llr.argtypes = [C.c_uint64, C.c_uint64, C.c_size_t, vp, C.c_size_t, vp]
+    lib.sogrand_gen_llr_device.argtypes = [vp, C.c_uint64, C.c_uint64, C.c_size_t, vp, C.c_size_t,
+                                           vp, vp]
+    _lib = lib
+    return lib
+
+
+def _strerror(code: int) -> str:
+    try:
+        return load_library().sogrand_strerror(code).decode()
+    except ImportError:
+        return "error"
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != OK:
+        raise SograndError(rc, what)
+
+
+def default_params(**kw) -> Params:
+    p = Params()
+    load_library().sogrand_params_default(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def _ptr(a) -> int | None:
+    """Address of a numpy array or torch tensor (host or device), or None."""
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        if not a.flags["C_CONTIGUOUS"]:
+            raise ValueError("arrays must be C-contiguous")
+        return a.ctypes.data
+    if hasattr(a, "data_ptr"):
+        if not a.is_contiguous():
+            raise ValueError("tensors must be contiguous")
+        return a.data_ptr()
+    raise TypeError(type(a))
+
+
+@dataclass
+class DecodeResult:
+    bits: object       # (n, 64) uint8, packed MSB-first
+    stats: object      # (n,) CW_STATS_DTYPE, or None
+    soft: object       # (n, 4096) float64 final L_APP, or None
+
+
+class Decoder:
+    """A decoder context on one gfx950 device (sogrand_ctx)."""
+
+    def __init__(self, device: int = 0, **params):
+        lib = load_library()
+        self._lib = lib
+        self.params = default_params(**params)
+        h = C.c_void_p()
+        _check(lib.sogrand_ctx_create(C.byref(h), device, C.byref(self.params)), "ctx_create")
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.sogrand_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_params(self, **kw) -> None:
+        p = Params.from_buffer_copy(self.params)
+        for k, v in kw.items():
+            setattr(p, k, v)
+        _check(self._lib.sogrand_ctx_set_params(self._h, C.byref(p)), "set_params")
+        self.params = p
+
+    def decode(self, llr, soft: bool = False, stats: bool = True, out_bits=None, out_stats=None,
+               out_soft=None, stream=None) -> DecodeResult:
+        """Decode cubes. llr: (n, 4096) float64, host (numpy) or device (torch) memory.
+        Outputs are allocated like the input (numpy for numpy) unless given."""
+        if isinstance(llr, np.ndarray):
+            llr = np.ascontiguousarray(llr, dtype=np.float64).reshape(-1, 4096)
+            n = llr.shape[0]
+            bits = out_bits if out_bits is not None else np.zeros((n, 64), np.uint8)
+            st = out_stats if out_stats is not None else (np.zeros(n, CW_STATS_DTYPE) if stats else None)
+            sf = out_soft if out_soft is not None else (np.zeros((n, 4096)) if soft else None)
+        else:
+            n = llr.numel() // 4096
+            bits, st, sf = out_bits, out_stats, out_soft
+            if bits is None:
+                raise ValueError("device decode needs out_bits (and optional out_stats/out_soft)")
+        rc = self._lib.sogrand_decode_cubic(self._h, _ptr(llr), n, _ptr(bits), _ptr(sf), _ptr(st),
+                                            stream)
+        _check(rc, "decode_cubic")
+        return DecodeResult(bits, st, sf)
+
+    def bitso(self, lines):
+        """Component SISO on (n, 16) lines -> (L_APP, L_E, N_guess)."""
+        lines = np.ascontiguousarray(lines, dtype=np.float64).reshape(-1, 16)
+        n = lines.shape[0]
+        la = np.zeros((n, 16))
+        le = np.zeros((n, 16))
+        ng = np.zeros(n, np.uint32)
+        rc = self._lib.sogrand_bitso_lines(self._h, lines.ctypes.data, n, la.ctypes.data,
+                                           le.ctypes.data, ng.ctypes.data, None)
+        _check(rc, "bitso_lines")
+        return la, le, ng
+
+    def last_kernel_ms(self) -> tuple[float, int]:
+        ms = C.c_double()
+        k = C.c_uint32()
+        _check(self._lib.sogrand_last_kernel_ms(self._h, C.byref(ms), C.byref(k)), "last_kernel_ms")
+        return ms.value, k.value
+
+    def gen_llr_device(self, seed: int, cw_offset: int, n: int, sigma, out, stream=None) -> None:
+        sig = np.ascontiguousarray(np.atleast_1d(sigma), dtype=np.float64) if not hasattr(
+            sigma, "data_ptr") else sigma
+        ns = sig.shape[0] if isinstance(sig, np.ndarray) else sig.numel()
+        _check(self._lib.sogrand_gen_llr_device(self._h, seed, cw_offset, n, _ptr(sig), ns,
+                                                _ptr(out), stream), "gen_llr_device")
+
+
+# ---- host-side data helpers (no GPU needed) ----
+
+def sigma_for_ebn0(ebn0_db: float) -> float:
+    return load_library().sogrand_sigma_for_ebn0(ebn0_db)
+
+
+def gen_info_bits(seed: int, n: int, cw_offset: int = 0) -> np.ndarray:
+    out = np.zeros((n, 64), np.uint8)
+    _check(load_library().sogrand_gen_info_bits(seed, cw_offset, n, out.ctypes.data), "gen_info")
+    return out
+
+
+def encode_cubes(info_bits: np.ndarray) -> np.ndarray:
+    info_bits = np.ascontiguousarray(info_bits, dtype=np.uint8).reshape(-1, 64)
+    out = np.zeros((info_bits.shape[0], 512), np.uint8)
+    _check(load_library().sogrand_encode_cubes(info_bits.ctypes.data, info_bits.shape[0],
+                                               out.ctypes.data), "encode")
+    return out
+
+
+def gen_llr(seed: int, n: int, ebn0_db=None, sigma=None, cw_offset: int = 0) -> np.ndarray:
+    """LLRs of n codewords (host). Give ebn0_db (scalar or per-codeword array) or sigma."""
+    if sigma is None:
+        e = np.atleast_1d(np.asarray(ebn0_db, dtype=np.float64))
+        sigma = np.array([sigma_for_ebn0(float(x)) for x in e])
+    sig = np.ascontiguousarray(np.atleast_1d(sigma), dtype=np.float64)
+    out = np.zeros((n, 4096))
+    _check(load_library().sogrand_gen_llr(seed, cw_offset, n, sig.ctypes.data, sig.shape[0],
+                                          out.ctypes.data), "gen_llr")
+    return out

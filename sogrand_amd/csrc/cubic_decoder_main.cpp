@@ -1,0 +1,156 @@
+// Drop-in replacement for the reference decoder executable:
+//
+//   cubic_decoder <input_llr_file> <output_file> [options]
+//
+// Same positional arguments, file formats and summary lines as
+// SOGRAND_C/cubic_decoder_final.c:62-314 and SOGRAND_CUDA/cubic_decoder1.cu:927-1183,
+// so it slots into cubic_flow.sh unchanged. Options (after the two paths):
+//   --tmax N            guess budget per component decode (default: unlimited)
+//   --precision P       fp64 (default, bit-exact with SOGRAND_C) | fp32
+//   --backend B         persistent (default) | plain
+//   --stats FILE        per-codeword statistics, 24-byte records
+//                       {u32 half_iters, u32 converged, u64 NG, u64 NG_p}
+//   --soft-out FILE     final L_APP, 4096 float64 per codeword
+//   --device N          HIP device ordinal (default 0)
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "sogrand.h"
+
+static int usage(const char *argv0)
+{
+    std::fprintf(stderr, "Usage: %s <input_llr_file> <output_file>\n", argv0);
+    return 1;
+}
+
+int main(int argc, char *argv[])
+{
+    if (argc < 3) return usage(argv[0]);
+    const char *in_path = argv[1];
+    const char *out_path = argv[2];
+    sogrand_params p;
+    sogrand_params_default(&p);
+    const char *stats_path = nullptr, *soft_path = nullptr;
+    int device = 0;
+    for (int i = 3; i < argc; ++i) {
+        std::string a = argv[i];
+        auto need = [&](void) -> const char * {
+            if (i + 1 >= argc) {
+                std::fprintf(stderr, "missing value for %s\n", a.c_str());
+                std::exit(1);
+            }
+            return argv[++i];
+        };
+        if (a == "--tmax") {
+            p.Tmax = std::strtoull(need(), nullptr, 10);
+        } else if (a == "--precision") {
+            std::string v = need();
+            if (v == "fp64") p.precision = SOGRAND_FP64;
+            else if (v == "fp32") p.precision = SOGRAND_FP32_MIXED;
+            else return usage(argv[0]);
+        } else if (a == "--backend") {
+            std::string v = need();
+            if (v == "plain") p.backend = SOGRAND_BACKEND_PLAIN;
+            else if (v == "persistent") p.backend = SOGRAND_BACKEND_PERSISTENT;
+            else return usage(argv[0]);
+        } else if (a == "--stats") {
+            stats_path = need();
+        } else if (a == "--soft-out") {
+            soft_path = need();
+        } else if (a == "--device") {
+            device = std::atoi(need());
+        } else {
+            return usage(argv[0]);
+        }
+    }
+
+    FILE *fin = std::fopen(in_path, "rb");
+    if (!fin) {
+        std::perror("Error opening input file");
+        return 1;
+    }
+    std::fseek(fin, 0, SEEK_END);
+    const long long file_size = std::ftell(fin);
+    std::rewind(fin);
+    // a trailing partial codeword is ignored (cubic_decoder1.cu:997, final.c:119)
+    const size_t total = (size_t)(file_size / (long long)(SOGRAND_CUBE * sizeof(double)));
+    if (total == 0) {
+        std::fprintf(stderr, "Error: No valid blocks found in input file\n");
+        std::fclose(fin);
+        return 1;
+    }
+    FILE *fout = std::fopen(out_path, "wb");
+    if (!fout) {
+        std::perror("Error opening output file");
+        std::fclose(fin);
+        return 1;
+    }
+    FILE *fstats = stats_path ? std::fopen(stats_path, "wb") : nullptr;
+    FILE *fsoft = soft_path ? std::fopen(soft_path, "wb") : nullptr;
+    if ((stats_path && !fstats) || (soft_path && !fsoft)) {
+        std::perror("Error opening statistics/soft output file");
+        return 1;
+    }
+
+    sogrand_ctx *ctx = nullptr;
+    int rc = sogrand_ctx_create(&ctx, device, &p);
+    if (rc) {
+        std::fprintf(stderr, "Error: %s\n", sogrand_strerror(rc));
+        return 1;
+    }
+    const size_t chunk = 65536;
+    std::vector<double> llr;
+    std::vector<uint8_t> bits;
+    std::vector<sogrand_cw_stats> st;
+    std::vector<double> soft;
+    uint64_t total_ng = 0, total_ngp = 0, total_half = 0;
+    double gpu_ms = 0.0;
+    for (size_t off = 0; off < total; off += chunk) {
+        const size_t n = (total - off) < chunk ? (total - off) : chunk;
+        llr.resize(n * SOGRAND_CUBE);
+        bits.resize(n * SOGRAND_INFO_BYTES);
+        st.resize(n);
+        if (fsoft) soft.resize(n * SOGRAND_CUBE);
+        if (std::fread(llr.data(), sizeof(double), n * SOGRAND_CUBE, fin) != n * SOGRAND_CUBE) {
+            std::fprintf(stderr, "Error: Failed to read entire file\n");
+            return 1;
+        }
+        rc = sogrand_decode_cubic(ctx, llr.data(), n, bits.data(), fsoft ? soft.data() : nullptr,
+                                  st.data(), nullptr);
+        if (rc) {
+            std::fprintf(stderr, "Error: %s\n", sogrand_strerror(rc));
+            return 1;
+        }
+        double ms = 0.0;
+        sogrand_last_kernel_ms(ctx, &ms, nullptr);
+        gpu_ms += ms;
+        for (size_t i = 0; i < n; ++i) {
+            total_ng += st[i].NG;
+            total_ngp += st[i].NG_p;
+            total_half += st[i].half_iters;
+        }
+        std::fwrite(bits.data(), 1, bits.size(), fout);
+        if (fstats) std::fwrite(st.data(), sizeof(sogrand_cw_stats), n, fstats);
+        if (fsoft) std::fwrite(soft.data(), sizeof(double), soft.size(), fsoft);
+    }
+    std::fclose(fin);
+    std::fclose(fout);
+    if (fstats) std::fclose(fstats);
+    if (fsoft) std::fclose(fsoft);
+    sogrand_ctx_destroy(ctx);
+
+    const double blocks = (double)total;
+    const double gpu_s = gpu_ms / 1000.0;
+    std::printf("Decoding complete. %zu block(s) decoded.\n", total);
+    std::printf("Average iterations per block: %.2f\n", (double)total_half / 2.0 / blocks);
+    std::printf("Average NG per block: %.2f\n", (double)total_ng / blocks);
+    std::printf("Average NG per info bit: %.2f\n", (double)total_ng / (blocks * 512.0));
+    std::printf("Average NG_p per block: %.2f\n", (double)total_ngp / blocks);
+    std::printf("Total GPU time: %.2f seconds\n", gpu_s);
+    std::printf("Throughput: %.2f blocks/sec\n", gpu_s > 0 ? blocks / gpu_s : 0.0);
+    return 0;
+}

@@ -1,0 +1,483 @@
+// C ABI of the SOGRAND gfx950 decoder (include/sogrand.h).
+#include "sogrand.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <vector>
+
+#include "sg_common.h"
+#include "sg_launch.h"
+#include "sg_schedule.h"
+
+#pragma clang fp contract(off)
+
+namespace {
+
+template <class T> struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    hipError_t reserve(size_t want)
+    {
+        if (want <= n) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        hipError_t e = hipMalloc(&p, sizeof(T) * want);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return e;
+        }
+        n = want;
+        return hipSuccess;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+bool is_device_ptr(const void *ptr)
+{
+    if (!ptr) return false;
+    hipPointerAttribute_t a;
+    hipError_t e = hipPointerGetAttributes(&a, ptr);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
+}
+
+} // namespace
+
+struct sogrand_ctx {
+    int device = 0;
+    sogrand_params p{};
+    hipStream_t own_stream = nullptr;
+    uint16_t *d_sched = nullptr;
+    DevBuf<double> le_scratch, llr_stage, soft_stage, sigma_stage;
+    DevBuf<uint8_t> info_stage;
+    DevBuf<uint32_t> stats_stage, queue;
+    DevBuf<int32_t> status;
+    DevBuf<uint32_t> nguess_stage;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0.0;
+    uint32_t last_launches = 0;
+    int cu_count = 256;
+    std::mutex mu;
+};
+
+extern "C" {
+
+void sogrand_params_default(sogrand_params *p)
+{
+    if (!p) return;
+    std::memset(p, 0, sizeof *p);
+    p->L = 3;
+    p->Imax = 30;
+    p->Tmax = UINT64_MAX;
+    p->p_ET = 1e-5;
+    p->alpha = 0.7;
+    p->precision = SOGRAND_FP64;
+    p->backend = SOGRAND_BACKEND_PERSISTENT;
+    p->wave_slots = 0;
+    p->waves_per_cu = 0;
+}
+
+const char *sogrand_strerror(int code)
+{
+    switch (code) {
+    case SOGRAND_OK: return "ok";
+    case SOGRAND_EINVAL: return "invalid argument";
+    case SOGRAND_ENOMEM: return "out of memory";
+    case SOGRAND_EHIP: return "HIP runtime error";
+    case SOGRAND_ENONFINITE: return "LLR input is NaN or infinite";
+    case SOGRAND_ENODEV: return "no gfx950 HIP device";
+    case SOGRAND_EIO: return "file I/O error";
+    case SOGRAND_EBUSY: return "admission refused";
+    default: return "unknown error";
+    }
+}
+
+static int check_params(const sogrand_params *p)
+{
+    if (!p) return SOGRAND_EINVAL;
+    if (p->L < 1 || p->L > SOGRAND_MAX_L) return SOGRAND_EINVAL;
+    if (p->Imax < 1 || p->Imax > 1000) return SOGRAND_EINVAL;
+    if (!(p->p_ET >= 0.0 && p->p_ET <= 1.0)) return SOGRAND_EINVAL;
+    if (!std::isfinite(p->alpha)) return SOGRAND_EINVAL;
+    if (p->precision != SOGRAND_FP64 && p->precision != SOGRAND_FP32_MIXED) return SOGRAND_EINVAL;
+    if (p->backend != SOGRAND_BACKEND_PLAIN && p->backend != SOGRAND_BACKEND_PERSISTENT)
+        return SOGRAND_EINVAL;
+    if (p->wave_slots < 0 || p->wave_slots > 8 || p->waves_per_cu < 0 || p->waves_per_cu > 32)
+        return SOGRAND_EINVAL;
+    return SOGRAND_OK;
+}
+
+int sogrand_ctx_create(sogrand_ctx **out, int device, const sogrand_params *params)
+{
+    if (!out) return SOGRAND_EINVAL;
+    *out = nullptr;
+    sogrand_params p;
+    if (params)
+        p = *params;
+    else
+        sogrand_params_default(&p);
+    int rc = check_params(&p);
+    if (rc) return rc;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) {
+        (void)hipGetLastError();
+        return SOGRAND_ENODEV;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return SOGRAND_EHIP;
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        std::fprintf(stderr, "sogrand: device %d is %s; this library is built for gfx950 only\n",
+                     device, prop.gcnArchName);
+        return SOGRAND_ENODEV;
+    }
+    sogrand_ctx *c = new (std::nothrow) sogrand_ctx;
+    if (!c) return SOGRAND_ENOMEM;
+    c->device = device;
+    c->p = p;
+    c->cu_count = prop.multiProcessorCount;
+    if (hipSetDevice(device) != hipSuccess) {
+        delete c;
+        return SOGRAND_EHIP;
+    }
+    if (hipStreamCreate(&c->own_stream) != hipSuccess || hipEventCreate(&c->ev0) != hipSuccess ||
+        hipEventCreate(&c->ev1) != hipSuccess) {
+        sogrand_ctx_destroy(c);
+        return SOGRAND_EHIP;
+    }
+    // one zero chunk of padding after the last table: the search loop
+    // prefetches one 16-byte chunk past the end marker
+    std::vector<uint16_t> tables((size_t)sg::kSchedTables * sg::kSchedStride + 8, 0);
+    sg::build_all_schedule_tables(tables.data());
+    if (hipMalloc(&c->d_sched, tables.size() * sizeof(uint16_t)) != hipSuccess) {
+        sogrand_ctx_destroy(c);
+        return SOGRAND_ENOMEM;
+    }
+    if (hipMemcpy(c->d_sched, tables.data(), tables.size() * sizeof(uint16_t),
+                  hipMemcpyHostToDevice) != hipSuccess) {
+        sogrand_ctx_destroy(c);
+        return SOGRAND_EHIP;
+    }
+    *out = c;
+    return SOGRAND_OK;
+}
+
+int sogrand_ctx_set_params(sogrand_ctx *ctx, const sogrand_params *params)
+{
+    if (!ctx) return SOGRAND_EINVAL;
+    int rc = check_params(params);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    ctx->p = *params;
+    return SOGRAND_OK;
+}
+
+void sogrand_ctx_destroy(sogrand_ctx *ctx)
+{
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->own_stream) (void)hipStreamSynchronize(ctx->own_stream);
+    ctx->le_scratch.release();
+    ctx->llr_stage.release();
+    ctx->soft_stage.release();
+    ctx->sigma_stage.release();
+    ctx->info_stage.release();
+    ctx->stats_stage.release();
+    ctx->queue.release();
+    ctx->status.release();
+    ctx->nguess_stage.release();
+    if (ctx->d_sched) (void)hipFree(ctx->d_sched);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
+    delete ctx;
+}
+
+static uint64_t effective_tmax(uint64_t t) { return t == 0 ? 0x7fffffffull : t; }
+
+// Decode one chunk whose buffers are all on the device.
+static int decode_device_chunk(sogrand_ctx *ctx, const double *d_llr, size_t n, uint8_t *d_info,
+                               double *d_soft, uint32_t *d_stats, hipStream_t st, float *ms)
+{
+    const sogrand_params &p = ctx->p;
+    if (ctx->status.reserve(n) != hipSuccess) return SOGRAND_ENOMEM;
+    sg::CubeJob job{};
+    job.llr = d_llr;
+    job.info = d_info;
+    job.soft = d_soft;
+    job.stats = d_stats;
+    job.status = ctx->status.p;
+    job.n_cw = n;
+    job.sched = ctx->d_sched;
+    job.tmax = effective_tmax(p.Tmax);
+    job.thres = 1.0 - p.p_ET;
+    job.alpha = p.alpha;
+    job.L = p.L;
+    job.Imax = p.Imax;
+    hipError_t e;
+    if (p.backend == SOGRAND_BACKEND_PERSISTENT) {
+        const int slots = p.wave_slots > 0 ? p.wave_slots : 2;
+        const int wpc = p.waves_per_cu > 0 ? p.waves_per_cu : 12;
+        const int wpb = 4; // waves per workgroup of the persistent kernel
+        int blocks = (ctx->cu_count * wpc + wpb - 1) / wpb;
+        // never more workers than cubes
+        const size_t max_blocks = (n * 1 + (size_t)wpb * slots - 1) / ((size_t)wpb * slots) + 1;
+        if ((size_t)blocks > max_blocks) blocks = (int)max_blocks;
+        const size_t total_slots = (size_t)blocks * wpb * slots;
+        if (ctx->le_scratch.reserve(total_slots * sg::kCube) != hipSuccess) return SOGRAND_ENOMEM;
+        if (ctx->queue.reserve(4) != hipSuccess) return SOGRAND_ENOMEM;
+        job.le_scratch = ctx->le_scratch.p;
+        job.queue = ctx->queue.p;
+        if (sg::launch_fill_u32(ctx->queue.p, 0u, 4, st) != hipSuccess) return SOGRAND_EHIP;
+        (void)hipEventRecord(ctx->ev0, st);
+        e = sg::launch_persistent(job, wpb, blocks, slots, st);
+    } else {
+        const size_t slots = sg::plain_scratch_slots(n);
+        if (ctx->le_scratch.reserve(slots * sg::kCube) != hipSuccess) return SOGRAND_ENOMEM;
+        job.le_scratch = ctx->le_scratch.p;
+        (void)hipEventRecord(ctx->ev0, st);
+        e = sg::launch_plain(job, (int)slots, st);
+    }
+    if (e != hipSuccess) return SOGRAND_EHIP;
+    (void)hipEventRecord(ctx->ev1, st);
+    if (hipEventSynchronize(ctx->ev1) != hipSuccess) return SOGRAND_EHIP;
+    float t = 0.f;
+    (void)hipEventElapsedTime(&t, ctx->ev0, ctx->ev1);
+    *ms += t;
+    std::vector<int32_t> status(n);
+    if (hipMemcpy(status.data(), ctx->status.p, n * sizeof(int32_t), hipMemcpyDeviceToHost) !=
+        hipSuccess)
+        return SOGRAND_EHIP;
+    for (size_t i = 0; i < n; ++i)
+        if (status[i]) return SOGRAND_ENONFINITE;
+    return SOGRAND_OK;
+}
+
+int sogrand_decode_cubic(sogrand_ctx *ctx, const double *llr, size_t n_cw, uint8_t *info_bits,
+                         double *soft_out, sogrand_cw_stats *stats, void *hip_stream)
+{
+    if (!ctx || (!llr && n_cw) || (!info_bits && n_cw)) return SOGRAND_EINVAL;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    ctx->last_ms = 0.0;
+    ctx->last_launches = 0;
+    if (n_cw == 0) return SOGRAND_OK;
+    if (hipSetDevice(ctx->device) != hipSuccess) return SOGRAND_EHIP;
+    hipStream_t st = hip_stream ? (hipStream_t)hip_stream : ctx->own_stream;
+    static_assert(sizeof(sogrand_cw_stats) == 24, "stats layout");
+
+    const bool dev_llr = is_device_ptr(llr), dev_info = is_device_ptr(info_bits);
+    const bool dev_soft = soft_out && is_device_ptr(soft_out);
+    const bool dev_stats = stats && is_device_ptr(stats);
+    const bool all_dev = dev_llr && dev_info && (!soft_out || dev_soft) && (!stats || dev_stats);
+    float ms = 0.f;
+    int rc = SOGRAND_OK;
+    if (all_dev) {
+        rc = decode_device_chunk(ctx, llr, n_cw, info_bits, soft_out, (uint32_t *)stats, st, &ms);
+        ctx->last_launches = 1;
+    } else {
+        const size_t chunk = 65536;
+        for (size_t off = 0; off < n_cw && rc == SOGRAND_OK; off += chunk) {
+            const size_t n = (n_cw - off) < chunk ? (n_cw - off) : chunk;
+            const double *d_llr = llr + off * sg::kCube;
+            if (!dev_llr) {
+                if (ctx->llr_stage.reserve(n * sg::kCube) != hipSuccess) return SOGRAND_ENOMEM;
+                if (hipMemcpyAsync(ctx->llr_stage.p, d_llr, n * sg::kCube * sizeof(double),
+                                   hipMemcpyHostToDevice, st) != hipSuccess)
+                    return SOGRAND_EHIP;
+                d_llr = ctx->llr_stage.p;
+            }
+            uint8_t *d_info = info_bits + off * sg::kInfoBytes;
+            if (!dev_info) {
+                if (ctx->info_stage.reserve(n * sg::kInfoBytes) != hipSuccess) return SOGRAND_ENOMEM;
+                d_info = ctx->info_stage.p;
+            }
+            double *d_soft = nullptr;
+            if (soft_out) {
+                d_soft = soft_out + off * sg::kCube;
+                if (!dev_soft) {
+                    if (ctx->soft_stage.reserve(n * sg::kCube) != hipSuccess) return SOGRAND_ENOMEM;
+                    d_soft = ctx->soft_stage.p;
+                }
+            }
+            uint32_t *d_stats = nullptr;
+            if (stats) {
+                d_stats = (uint32_t *)(stats + off);
+                if (!dev_stats) {
+                    if (ctx->stats_stage.reserve(n * 6) != hipSuccess) return SOGRAND_ENOMEM;
+                    d_stats = ctx->stats_stage.p;
+                }
+            }
+            rc = decode_device_chunk(ctx, d_llr, n, d_info, d_soft, d_stats, st, &ms);
+            ctx->last_launches += 1;
+            if (rc != SOGRAND_OK) break;
+            if (!dev_info &&
+                hipMemcpy(info_bits + off * sg::kInfoBytes, d_info, n * sg::kInfoBytes,
+                          hipMemcpyDeviceToHost) != hipSuccess)
+                return SOGRAND_EHIP;
+            if (soft_out && !dev_soft &&
+                hipMemcpy(soft_out + off * sg::kCube, d_soft, n * sg::kCube * sizeof(double),
+                          hipMemcpyDeviceToHost) != hipSuccess)
+                return SOGRAND_EHIP;
+            if (stats && !dev_stats &&
+                hipMemcpy(stats + off, d_stats, n * sizeof(sogrand_cw_stats),
+                          hipMemcpyDeviceToHost) != hipSuccess)
+                return SOGRAND_EHIP;
+        }
+    }
+    ctx->last_ms = ms;
+    return rc;
+}
+
+int sogrand_bitso_lines(sogrand_ctx *ctx, const double *llr, size_t n_lines, double *L_APP,
+                        double *L_E, uint32_t *N_guess, void *hip_stream)
+{
+    if (!ctx || (!llr && n_lines)) return SOGRAND_EINVAL;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    if (n_lines == 0) return SOGRAND_OK;
+    if (hipSetDevice(ctx->device) != hipSuccess) return SOGRAND_EHIP;
+    hipStream_t st = hip_stream ? (hipStream_t)hip_stream : ctx->own_stream;
+    // stage everything through one device buffer: llr | la | le
+    if (ctx->llr_stage.reserve(n_lines * 16 * 3) != hipSuccess) return SOGRAND_ENOMEM;
+    if (ctx->nguess_stage.reserve(n_lines) != hipSuccess) return SOGRAND_ENOMEM;
+    if (ctx->status.reserve(n_lines) != hipSuccess) return SOGRAND_ENOMEM;
+    double *d_in = ctx->llr_stage.p, *d_la = d_in + n_lines * 16, *d_le = d_la + n_lines * 16;
+    if (hipMemcpyAsync(d_in, llr, n_lines * 16 * sizeof(double), hipMemcpyDefault, st) !=
+        hipSuccess)
+        return SOGRAND_EHIP;
+    if (sg::launch_lines(d_in, n_lines, d_la, d_le, ctx->nguess_stage.p, ctx->status.p,
+                         ctx->d_sched, effective_tmax(ctx->p.Tmax), 1.0 - ctx->p.p_ET, ctx->p.L,
+                         st) != hipSuccess)
+        return SOGRAND_EHIP;
+    if (hipStreamSynchronize(st) != hipSuccess) return SOGRAND_EHIP;
+    if (L_APP && hipMemcpy(L_APP, d_la, n_lines * 16 * sizeof(double), hipMemcpyDefault) !=
+                     hipSuccess)
+        return SOGRAND_EHIP;
+    if (L_E &&
+        hipMemcpy(L_E, d_le, n_lines * 16 * sizeof(double), hipMemcpyDefault) != hipSuccess)
+        return SOGRAND_EHIP;
+    if (N_guess && hipMemcpy(N_guess, ctx->nguess_stage.p, n_lines * sizeof(uint32_t),
+                             hipMemcpyDefault) != hipSuccess)
+        return SOGRAND_EHIP;
+    std::vector<int32_t> status(n_lines);
+    if (hipMemcpy(status.data(), ctx->status.p, n_lines * sizeof(int32_t),
+                  hipMemcpyDeviceToHost) != hipSuccess)
+        return SOGRAND_EHIP;
+    for (size_t i = 0; i < n_lines; ++i)
+        if (status[i]) return SOGRAND_ENONFINITE;
+    return SOGRAND_OK;
+}
+
+int sogrand_last_kernel_ms(sogrand_ctx *ctx, double *ms, uint32_t *launches)
+{
+    if (!ctx) return SOGRAND_EINVAL;
+    if (ms) *ms = ctx->last_ms;
+    if (launches) *launches = ctx->last_launches;
+    return SOGRAND_OK;
+}
+
+// ---- synthetic data ---------------------------------------------------------
+
+double sogrand_sigma_for_ebn0(double ebn0_db)
+{
+    const double R = std::pow(8.0 / 16.0, 3.0); // channel_sim.c:32
+    const double esn0 = ebn0_db + 10.0 * std::log10(2.0 * R);
+    return 1.0 / std::sqrt(std::pow(10.0, esn0 / 10.0));
+}
+
+int sogrand_gen_info_bits(uint64_t seed, uint64_t cw_offset, size_t n_cw, uint8_t *info_bits)
+{
+    if (!info_bits && n_cw) return SOGRAND_EINVAL;
+    for (size_t c = 0; c < n_cw; ++c)
+        for (int j = 0; j < 8; ++j) {
+            const uint64_t w = sg::info_word(seed, cw_offset + c, j);
+            for (int q = 0; q < 8; ++q) info_bits[c * 64 + 8 * j + q] = (uint8_t)(w >> (8 * q));
+        }
+    return SOGRAND_OK;
+}
+
+int sogrand_encode_cubes(const uint8_t *info_bits, size_t n_cw, uint8_t *code_bytes)
+{
+    if ((!info_bits || !code_bytes) && n_cw) return SOGRAND_EINVAL;
+    uint32_t W[256];
+    for (size_t c = 0; c < n_cw; ++c) {
+        sg::encode_cube_words(info_bits + c * 64, W);
+        uint8_t *o = code_bytes + c * 512;
+        for (int w = 0; w < 256; ++w) { // bit idx = 16*w + a0, MSB first
+            o[2 * w] = (uint8_t)sg::rev8(W[w] & 0xffu);
+            o[2 * w + 1] = (uint8_t)sg::rev8((W[w] >> 8) & 0xffu);
+        }
+    }
+    return SOGRAND_OK;
+}
+
+int sogrand_gen_llr(uint64_t seed, uint64_t cw_offset, size_t n_cw, const double *sigma,
+                    size_t n_sigma, double *llr)
+{
+    if ((!sigma || !llr) && n_cw) return SOGRAND_EINVAL;
+    if (n_cw && n_sigma != 1 && n_sigma != n_cw) return SOGRAND_EINVAL;
+    uint8_t info[64];
+    uint32_t W[256];
+    for (size_t c = 0; c < n_cw; ++c) {
+        const uint64_t cw = cw_offset + c;
+        for (int j = 0; j < 8; ++j) {
+            const uint64_t w = sg::info_word(seed, cw, j);
+            for (int q = 0; q < 8; ++q) info[8 * j + q] = (uint8_t)(w >> (8 * q));
+        }
+        sg::encode_cube_words(info, W);
+        const double sgm = sigma[n_sigma == 1 ? 0 : c];
+        const double var = sgm * sgm;
+        double *out = llr + c * sg::kCube;
+        for (int k = 0; k < 2048; ++k) {
+            double z0, z1;
+            sg::normal_pair(seed, cw, (uint64_t)k, &z0, &z1);
+            const int p0 = 2 * k, p1 = 2 * k + 1;
+            const double x0 = ((W[p0 >> 4] >> (p0 & 15)) & 1u) ? -1.0 : 1.0;
+            const double x1 = ((W[p1 >> 4] >> (p1 & 15)) & 1u) ? -1.0 : 1.0;
+            const double n0 = sgm * z0, n1 = sgm * z1;
+            const double y0 = x0 + n0, y1 = x1 + n1;
+            out[p0] = 2.0 * y0 / var;
+            out[p1] = 2.0 * y1 / var;
+        }
+    }
+    return SOGRAND_OK;
+}
+
+int sogrand_gen_llr_device(sogrand_ctx *ctx, uint64_t seed, uint64_t cw_offset, size_t n_cw,
+                           const double *sigma, size_t n_sigma, double *llr, void *hip_stream)
+{
+    if (!ctx || ((!sigma || !llr) && n_cw)) return SOGRAND_EINVAL;
+    if (n_cw && n_sigma != 1 && n_sigma != n_cw) return SOGRAND_EINVAL;
+    if (n_cw == 0) return SOGRAND_OK;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    if (hipSetDevice(ctx->device) != hipSuccess) return SOGRAND_EHIP;
+    hipStream_t st = hip_stream ? (hipStream_t)hip_stream : ctx->own_stream;
+    if (!is_device_ptr(llr)) return SOGRAND_EINVAL;
+    const double *d_sigma = sigma;
+    if (!is_device_ptr(sigma)) {
+        if (ctx->sigma_stage.reserve(n_sigma) != hipSuccess) return SOGRAND_ENOMEM;
+        if (hipMemcpyAsync(ctx->sigma_stage.p, sigma, n_sigma * sizeof(double),
+                           hipMemcpyHostToDevice, st) != hipSuccess)
+            return SOGRAND_EHIP;
+        d_sigma = ctx->sigma_stage.p;
+    }
+    if (sg::launch_gen(seed, cw_offset, n_cw, d_sigma, n_sigma, llr, st) != hipSuccess)
+        return SOGRAND_EHIP;
+    if (hipStreamSynchronize(st) != hipSuccess) return SOGRAND_EHIP;
+    return SOGRAND_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,349 @@
+// HIP kernels of the SOGRAND (16,8)^3 decoder for gfx950.
+//
+// plain_kernel   : one 256-lane workgroup per cube; lane t owns line t of every
+//                  phase; L_E in a per-workgroup global slot, hard decisions in
+//                  LDS; phases separated by workgroup barriers. Replaces the
+//                  reference GPU kernel highly_optimized_sogrand_kernel
+//                  (SOGRAND_CUDA/cubic_decoder1.cu:635-841) with the CPU
+//                  decoder's semantics (SOGRAND_C/cubic_decoder_final.c:119-289).
+// lines_kernel   : SOGRAND_bitSO (cubic_decoder_final.c:521) on independent lines.
+// gen_kernel     : seeded info bits -> product encoding -> BPSK/AWGN -> LLR.
+#include <hip/hip_runtime.h>
+
+#include "sg_common.h"
+#include "sg_device.h"
+#include "sg_launch.h"
+
+#pragma clang fp contract(off)
+
+namespace sg {
+
+struct CwStatsDev {
+    uint32_t half_iters;
+    uint32_t converged;
+    uint64_t NG;
+    uint64_t NG_p;
+};
+
+// Axis geometry: line (x, y) of axis ax covers base + j*stride, j = 0..15.
+// axis 0: (a1, a2) over a0; axis 1: (a0, a2) over a1; axis 2: (a0, a1) over a2.
+__device__ __forceinline__ void axis_line(int ax, int line, int &base, int &stride)
+{
+    const int x = line & 15, y = line >> 4;
+    if (ax == 0) {
+        base = 16 * x + 256 * y;
+        stride = 1;
+    } else if (ax == 1) {
+        base = x + 256 * y;
+        stride = 16;
+    } else {
+        base = x + 16 * y;
+        stride = 256;
+    }
+}
+
+// Component input of position p: L_channel + alpha * L_E, a multiply then an
+// add (:147-155). In the first phase L_E is all zeros (:129).
+struct CubeSrc {
+    const double *lch;
+    const double *le;
+    double alpha;
+    int base, stride;
+    bool first;
+    __device__ double operator()(int i) const
+    {
+        const int p = base + i * stride;
+        const double e = first ? 0.0 : le[p];
+        const double t = alpha * e;
+        return lch[p] + t;
+    }
+};
+
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        uint32_t w = __shfl_xor(v, o, 64);
+        v = w > v ? w : v;
+    }
+    return v;
+}
+
+constexpr int kPlainThreads = 256;
+
+template <bool FINITE_T>
+__global__ __launch_bounds__(kPlainThreads) void plain_kernel(CubeJob job)
+{
+    __shared__ double s_delta[16 * kPlainThreads];
+    __shared__ uint16_t s_hp[16 * kPlainThreads];
+    __shared__ uint8_t s_hd[kCube];
+    __shared__ uint32_t s_sum[kPlainThreads / 64], s_max[kPlainThreads / 64];
+
+    const int t = threadIdx.x;
+    const LineLds S{s_delta, s_hp, kPlainThreads, t};
+    const DecodeCfg cfg{job.tmax, job.thres, job.sched, job.L};
+    double *le = job.le_scratch + (size_t)blockIdx.x * kCube;
+
+    for (size_t cube = blockIdx.x; cube < job.n_cw; cube += gridDim.x) {
+        const double *lch = job.llr + cube * kCube;
+        double *soft = job.soft ? job.soft + cube * kCube : nullptr;
+        uint32_t half = 0, conv = 0;
+        uint64_t ng = 0, ngp = 0;
+        bool bad = false;
+        for (uint32_t it = 1; it <= job.Imax && !conv; ++it) {
+            for (int ax = 0; ax < 3; ++ax) {
+                ++half; // n_iter += 0.5 before each phase (:145, :184, :223)
+                CubeSrc src{lch, le, job.alpha, 0, 0, it == 1 && ax == 0};
+                axis_line(ax, t, src.base, src.stride);
+                uint32_t hd, s0;
+                double pm_hd;
+                int32_t ic;
+                LineList ls;
+                if (line_setup(src, S, hd, s0, pm_hd, ic)) {
+                    line_search<FINITE_T>(S, hd, s0, pm_hd, ic, cfg, ls);
+                } else {
+                    bad = true;
+                    ls.T = 0;
+                    ls.nl = 0;
+                    ls.pnl = 0.0;
+                }
+                const double pnotin = fmax(ls.pnl, 1e-40);
+#pragma unroll 1
+                for (int i = 0; i < 16; ++i) {
+                    const int p = src.base + i * src.stride;
+                    const double xin = src(i);
+                    double la, lenew;
+                    if (ls.nl == 0) { // empty list: L_APP = llr, L_E = 0 (:537-539)
+                        la = xin;
+                        lenew = 0.0;
+                    } else {
+                        la = soft_app(ls, pnotin, i, xin);
+                        lenew = la - xin;
+                    }
+                    le[p] = lenew;
+                    if (soft) soft[p] = la;
+                    s_hd[p] = (la > 0.0) ? 0 : 1;
+                }
+                // NG += N_guess; NG_p += max over the phase's lines (:162-163, :170)
+                const uint32_t ws = wave_sum_u32(ls.T), wm = wave_max_u32(ls.T);
+                if ((t & 63) == 0) {
+                    s_sum[t >> 6] = ws;
+                    s_max[t >> 6] = wm;
+                }
+                __syncthreads();
+                uint32_t bs = 0, bm = 0;
+#pragma unroll
+                for (int w = 0; w < kPlainThreads / 64; ++w) {
+                    bs += s_sum[w];
+                    bm = s_max[w] > bm ? s_max[w] : bm;
+                }
+                ng += bs;
+                ngp += bm;
+                // early termination: the hard decision is a product codeword iff
+                // all 768 lines are component codewords (equivalent to the
+                // re-encode test of :317-391)
+                bool line_bad = false;
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    int b0, st;
+                    axis_line(a, t, b0, st);
+                    uint32_t w = 0;
+#pragma unroll
+                    for (int j = 0; j < 16; ++j) w |= (uint32_t)s_hd[b0 + j * st] << j;
+                    line_bad = line_bad || (syndrome16(w) != 0);
+                }
+                const int any_bad = __syncthreads_or(line_bad);
+                if (!any_bad) {
+                    conv = 1;
+                    break;
+                }
+            }
+        }
+        if (t < kInfoBytes) { // info bit b = a0 + 8*a1 + 64*a2, MSB first (:267-280)
+            uint32_t byte = 0;
+            const int a1 = t & 7, a2 = t >> 3;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) byte |= (uint32_t)s_hd[q + 16 * a1 + 256 * a2] << (7 - q);
+            job.info[cube * kInfoBytes + t] = (uint8_t)byte;
+        }
+        if (t == 0) {
+            if (job.stats) {
+                CwStatsDev *st = reinterpret_cast<CwStatsDev *>(job.stats) + cube;
+                st->half_iters = half;
+                st->converged = conv;
+                st->NG = ng;
+                st->NG_p = ngp;
+            }
+        }
+        const int any = __syncthreads_or(bad);
+        if (t == 0) job.status[cube] = any ? 1 : 0;
+    }
+}
+
+template <bool FINITE_T>
+__global__ __launch_bounds__(256) void lines_kernel(const double *llr, size_t n, double *la,
+                                                    double *le, uint32_t *nguess,
+                                                    int32_t *status, DecodeCfg cfg)
+{
+    __shared__ double s_delta[16 * 256];
+    __shared__ uint16_t s_hp[16 * 256];
+    const size_t line = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (line >= n) return;
+    const LineLds S{s_delta, s_hp, 256, (int)threadIdx.x};
+    const double *x = llr + line * 16;
+    auto src = [x](int i) { return x[i]; };
+    uint32_t hd, s0;
+    double pm_hd;
+    int32_t ic;
+    LineList ls;
+    const bool ok = line_setup(src, S, hd, s0, pm_hd, ic);
+    if (ok) {
+        line_search<FINITE_T>(S, hd, s0, pm_hd, ic, cfg, ls);
+    } else {
+        ls.T = 0;
+        ls.nl = 0;
+        ls.pnl = 0.0;
+    }
+    const double pnotin = fmax(ls.pnl, 1e-40);
+#pragma unroll 1
+    for (int i = 0; i < 16; ++i) {
+        const double xi = x[i];
+        double a, e;
+        if (ls.nl == 0) {
+            a = xi;
+            e = 0.0;
+        } else {
+            a = soft_app(ls, pnotin, i, xi);
+            e = a - xi;
+        }
+        if (la) la[line * 16 + i] = a;
+        if (le) le[line * 16 + i] = e;
+    }
+    if (nguess) nguess[line] = ls.T;
+    status[line] = ok ? 0 : 1;
+}
+
+// One 256-thread workgroup per codeword: hash the info bits, encode the cube
+// in LDS, then each thread writes 16 LLRs (8 Box-Muller pairs).
+__global__ __launch_bounds__(256) void gen_kernel(uint64_t seed, uint64_t cw_offset, size_t n_cw,
+                                                  const double *sigma, size_t n_sigma, double *llr)
+{
+    __shared__ uint8_t s_info[kInfoBytes];
+    __shared__ uint32_t s_w[256];
+    const int t = threadIdx.x;
+    for (size_t c = blockIdx.x; c < n_cw; c += gridDim.x) {
+        const uint64_t cw = cw_offset + c;
+        if (t < 8) {
+            const uint64_t w = info_word(seed, cw, t);
+#pragma unroll
+            for (int q = 0; q < 8; ++q) s_info[8 * t + q] = (uint8_t)(w >> (8 * q));
+        }
+        __syncthreads();
+        if (t < 64) {
+            const int i1 = t & 7, i2 = t >> 3;
+            s_w[i1 + 16 * i2] = encode_row(rev8(s_info[i1 + 8 * i2]));
+        }
+        __syncthreads();
+        if (t < 64) {
+            const int a1 = 8 + (t & 7), i2 = t >> 3;
+            uint32_t acc = 0;
+            for (int i1 = 0; i1 < 8; ++i1)
+                if ((hcol_of((unsigned)i1) >> (a1 - 8)) & 1u) acc ^= s_w[i1 + 16 * i2];
+            s_w[a1 + 16 * i2] = acc;
+        }
+        __syncthreads();
+        if (t < 128) {
+            const int a1 = t & 15, a2 = 8 + (t >> 4);
+            uint32_t acc = 0;
+            for (int i2 = 0; i2 < 8; ++i2)
+                if ((hcol_of((unsigned)i2) >> (a2 - 8)) & 1u) acc ^= s_w[a1 + 16 * i2];
+            s_w[a1 + 16 * a2] = acc;
+        }
+        __syncthreads();
+        const double sg = sigma[n_sigma == 1 ? 0 : c];
+        const double var = sg * sg;
+        const uint32_t word = s_w[t];
+        double *out = llr + c * kCube + 16 * t;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            double z0, z1;
+            normal_pair(seed, cw, (uint64_t)(8 * t + q), &z0, &z1);
+            const int p0 = 2 * q, p1 = 2 * q + 1;
+            const double x0 = ((word >> p0) & 1u) ? -1.0 : 1.0;
+            const double x1 = ((word >> p1) & 1u) ? -1.0 : 1.0;
+            const double n0 = sg * z0, n1 = sg * z1;
+            const double y0 = x0 + n0, y1 = x1 + n1;
+            out[p0] = 2.0 * y0 / var;
+            out[p1] = 2.0 * y1 / var;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void fill_u32_kernel(uint32_t *p, uint32_t v, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+
+size_t plain_scratch_slots(size_t n_cw)
+{
+    const size_t cap = 131072;
+    return n_cw < cap ? n_cw : cap;
+}
+
+hipError_t launch_plain(const CubeJob &job, int grid, hipStream_t st)
+{
+    if (job.n_cw == 0) return hipSuccess;
+    if (job.tmax >= (1ull << 17))
+        hipLaunchKernelGGL(plain_kernel<false>, dim3(grid), dim3(kPlainThreads), 0, st, job);
+    else
+        hipLaunchKernelGGL(plain_kernel<true>, dim3(grid), dim3(kPlainThreads), 0, st, job);
+    return hipGetLastError();
+}
+
+hipError_t launch_lines(const double *llr, size_t n_lines, double *la, double *le,
+                        uint32_t *nguess, int32_t *status, const uint16_t *sched, uint64_t tmax,
+                        double thres, uint32_t L, hipStream_t st)
+{
+    if (n_lines == 0) return hipSuccess;
+    const DecodeCfg cfg{tmax, thres, sched, L};
+    const unsigned grid = (unsigned)((n_lines + 255) / 256);
+    if (tmax >= (1ull << 17))
+        hipLaunchKernelGGL(lines_kernel<false>, dim3(grid), dim3(256), 0, st, llr, n_lines, la, le,
+                           nguess, status, cfg);
+    else
+        hipLaunchKernelGGL(lines_kernel<true>, dim3(grid), dim3(256), 0, st, llr, n_lines, la, le,
+                           nguess, status, cfg);
+    return hipGetLastError();
+}
+
+hipError_t launch_gen(uint64_t seed, uint64_t cw_offset, size_t n_cw, const double *sigma,
+                      size_t n_sigma, double *llr, hipStream_t st)
+{
+    if (n_cw == 0) return hipSuccess;
+    const unsigned grid = (unsigned)(n_cw < 65536 ? n_cw : 65536);
+    hipLaunchKernelGGL(gen_kernel, dim3(grid), dim3(256), 0, st, seed, cw_offset, n_cw, sigma,
+                       n_sigma, llr);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    unsigned grid = (unsigned)((n + 255) / 256);
+    if (grid > 4096) grid = 4096;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(grid), dim3(256), 0, st, p, v, n);
+    return hipGetLastError();
+}
+
+} // namespace sg

@@ -1,0 +1,49 @@
+// Host-side launch interface between the C ABI (sg_api.cpp) and the HIP
+// kernels (sg_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace sg {
+
+struct CubeJob {
+    const double *llr;        // n_cw x 4096 (device)
+    uint8_t *info;            // n_cw x 64 (device)
+    double *soft;             // n_cw x 4096 or null (device)
+    uint32_t *stats;          // n_cw x 6 u32 {half, conv, NG lo, NG hi, NGp lo, NGp hi} or null
+    int32_t *status;          // n_cw: 0 ok, 1 non-finite input (device)
+    double *le_scratch;       // one 4096 f64 slot per resident workgroup / wave slot
+    uint32_t *queue;          // persistent: work counter (device), zeroed by the launcher
+    size_t n_cw;
+    const uint16_t *sched;
+    uint64_t tmax;
+    double thres;
+    double alpha;
+    uint32_t L;
+    uint32_t Imax;
+};
+
+// Plain backend: one 256-lane workgroup per cube, one lane per line.
+hipError_t launch_plain(const CubeJob &job, int grid, hipStream_t st);
+size_t plain_scratch_slots(size_t n_cw);
+
+// Persistent backend: waves pop cubes from a device counter; each wave keeps
+// `slots` cubes in flight and hands lines to lanes as they free up.
+hipError_t launch_persistent(const CubeJob &job, int waves_per_block, int blocks, int slots,
+                             hipStream_t st);
+
+// Line-level SISO (SOGRAND_bitSO on many lines).
+hipError_t launch_lines(const double *llr, size_t n_lines, double *la, double *le,
+                        uint32_t *nguess, int32_t *status, const uint16_t *sched, uint64_t tmax,
+                        double thres, uint32_t L, hipStream_t st);
+
+// Synthetic LLR generation on the device.
+hipError_t launch_gen(uint64_t seed, uint64_t cw_offset, size_t n_cw, const double *sigma,
+                      size_t n_sigma, double *llr, hipStream_t st);
+
+// Fill kernel (used instead of hipMemsetAsync so that callers may capture it).
+hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t st);
+
+} // namespace sg
