@@ -29,7 +29,7 @@ extern "C" {
 #define SOGRAND_CUBE 4096     /* LLRs per codeword          */
 #define SOGRAND_INFO_BITS 512 /* info bits per codeword     */
 #define SOGRAND_INFO_BYTES 64 /* packed info bytes per cw   */
-#define SOGRAND_MAX_L 4       /* largest list size supported */
+#define SOGRAND_MAX_L 3       /* largest list size supported (the reference uses 3) */
 
 /* error codes */
 #define SOGRAND_OK 0
@@ -61,8 +61,12 @@ typedef struct sogrand_params {
     double alpha;       /* extrinsic scaling                                             */
     int32_t precision;  /* SOGRAND_FP64 or SOGRAND_FP32_MIXED                            */
     int32_t backend;    /* SOGRAND_BACKEND_*                                             */
-    int32_t wave_slots; /* persistent: cube slots per wave (0 = tuned default)           */
-    int32_t waves_per_cu; /* persistent: resident waves per CU (0 = tuned default)       */
+    int32_t wave_slots; /* persistent: cube slots per wave, 1 or 2 (0 = tuned default)   */
+    int32_t waves_per_cu; /* persistent: resident waves per CU (0 = as many as fit)      */
+    int32_t lane_batch; /* persistent: free lanes gathered before new lines are handed out
+                           (0 = tuned default)                                            */
+    int32_t step_len;   /* persistent: guesses per lane between scheduling points
+                           (0 = tuned default)                                            */
 } sogrand_params;
 
 /* Per-codeword statistics (the reference prints only their averages,
@@ -106,6 +110,13 @@ int sogrand_bitso_lines(sogrand_ctx *ctx, const double *llr, size_t n_lines, dou
 /* Time (ms, HIP events on the launch stream) of the decode kernels of the last
  * sogrand_decode_cubic call on this context, and how many launches it made. */
 int sogrand_last_kernel_ms(sogrand_ctx *ctx, double *ms, uint32_t *launches);
+
+/* The error-pattern schedule the decoder streams for intercept ic (ic >= 55
+ * all share one order): 65535 16-bit masks over sorted positions (bit b = b-th
+ * least reliable bit) in the order the reference's landslide loop visits them
+ * (cubic_decoder_final.c:642-744, mountain_build :775-789), then a 0. out holds
+ * 65536 entries. Host only. */
+int sogrand_schedule_table(int32_t ic, uint16_t *out);
 
 /* ---- synthetic data, restating the reference's channel (channel_sim.c:34-39,61-89)
  * with a seeded counter-based generator. Codeword c of a stream depends only on

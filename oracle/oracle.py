@@ -13,7 +13,10 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "liboracle.so")
+ORACLE_FMA_SO = os.path.join(HERE, "liboracle_fma.so")
 REF_SO = os.path.join(HERE, "_ref", "libsogrand_ref.so")
+REF_SO_NATIVE = os.path.join(HERE, "_ref", "libsogrand_ref_native.so")
+REF_BIN_NATIVE = os.path.join(HERE, "_ref", "cubic_decoder_final_native")
 REF_BIN = os.path.join(HERE, "_ref", "cubic_decoder_final")
 REF_BIN_O2 = os.path.join(HERE, "_ref", "cubic_decoder_final_O2")
 REF_ENCODER = os.path.join(HERE, "_ref", "cubic_encoder")
@@ -53,7 +56,7 @@ CW_STATS_DTYPE = np.dtype(
 def build_oracle() -> None:
     import subprocess
 
-    subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], check=True)
+    subprocess.run(["make", "-s", "-C", HERE, "liboracle.so", "liboracle_fma.so"], check=True)
 
 
 def build_reference() -> bool:
@@ -85,6 +88,10 @@ class Oracle:
         lib.og_decode_cubes.argtypes = [_dp, C.c_size_t, C.c_int, C.c_int, C.c_uint64, C.c_double,
                                         C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
         lib.og_decode_cubes.restype = C.c_int
+        lib.og_decode_cubes_cond.argtypes = [_dp, C.c_size_t, C.c_int, C.c_int, C.c_uint64,
+                                             C.c_double, C.c_double, C.c_void_p, C.c_void_p,
+                                             C.c_void_p, C.c_void_p, C.c_int]
+        lib.og_decode_cubes_cond.restype = C.c_int
         self.lib = lib
 
     def code_tables(self):
@@ -112,24 +119,38 @@ class Oracle:
         tr = LineTrace()
         rc = self.lib.og_siso(llr, L, Tmax, thres, la.ctypes.data, le.ctypes.data, C.byref(tr))
         if rc:
-            raise ValueError("NaN LLR")
+            raise ValueError("non-finite LLR")
         return la, le, tr
+
+    @staticmethod
+    def chain_cond(tr) -> float:
+        """Relative rounding-error bound of the reference's P_notGuess for a line
+        (same definition as og_chain_cond)."""
+        if tr.n_list == 0:
+            return 0.0
+        P = 256.0 * tr.pnl
+        return (tr.T + 64.0) * 2.0 ** -50 / abs(P) if P != 0 else float("inf")
 
     def is_codeword(self, cube) -> bool:
         return bool(self.lib.og_cube_is_codeword(np.ascontiguousarray(cube, dtype=np.float64)))
 
     def decode(self, llr, L=3, Imax=30, Tmax=UINT64_MAX, p_et=1e-5, alpha=0.7, soft=False,
-               threads=0):
+               threads=0, cond=False):
+        """Decode cubes -> (bits, stats, soft or None) [+ cond if cond=True]."""
         llr = np.ascontiguousarray(llr, dtype=np.float64).reshape(-1, 4096)
         n = llr.shape[0]
         bits = np.zeros((n, 64), np.uint8)
         stats = np.zeros(n, CW_STATS_DTYPE)
         sarr = np.zeros((n, 4096)) if soft else None
-        rc = self.lib.og_decode_cubes(llr, n, L, Imax, Tmax, p_et, alpha, bits.ctypes.data,
-                                      sarr.ctypes.data if soft else None, stats.ctypes.data,
-                                      threads)
+        carr = np.zeros((n, 4096)) if cond else None
+        rc = self.lib.og_decode_cubes_cond(llr, n, L, Imax, Tmax, p_et, alpha, bits.ctypes.data,
+                                           sarr.ctypes.data if soft else None,
+                                           carr.ctypes.data if cond else None, stats.ctypes.data,
+                                           threads)
         if rc:
-            raise ValueError("NaN LLR")
+            raise ValueError("non-finite LLR")
+        if cond:
+            return bits, stats, sarr, carr
         return bits, stats, sarr
 
 

@@ -275,7 +275,7 @@ int og_siso(const double *llr, int L, uint64_t Tmax, double thres, double *L_APP
     unsigned hd = 0;
     uint8_t s0 = 0;
     for (int i = 0; i < 16; i++) {
-        if (isnan(llr[i])) return -1;
+        if (!isfinite(llr[i])) return -1; /* the decoders' contract: finite LLRs only */
         if (!(llr[i] > 0.0)) { /* HardDec (:758) */
             hd |= 1u << i;
             s0 ^= og_hcol[i];
@@ -444,14 +444,28 @@ typedef struct {
     double thres, alpha;
     uint8_t *bits;
     double *soft;
+    double *cond;
     og_cw_stats *stats;
     size_t next; /* shared work counter */
     int bad;
     pthread_mutex_t mu;
 } og_job;
 
+/* Rounding-error bound, relative, of the reference's P_notGuess at the last list
+ * insertion of a line: T guesses of magnitude <= 1 subtracted from 1.0, each
+ * with increments whose own error is a few ulps (T + 64) * 2^-50 in absolute
+ * terms, divided by the value. 0 for an empty list (L_APP = input exactly). */
+static double og_chain_cond(const og_line_trace *tr)
+{
+    if (tr->n_list == 0) return 0.0;
+    double P = 256.0 * tr->pnl;
+    if (!(fabs(P) > 0.0)) return INFINITY;
+    return ((double)tr->T + 64.0) * 8.8817841970012523e-16 / fabs(P); /* 2^-50 */
+}
+
 static int og_decode_one(const double *Lch, int L, int Imax, uint64_t Tmax, double thres,
-                         double alpha, uint8_t *bits, double *soft, og_cw_stats *st)
+                         double alpha, uint8_t *bits, double *soft, double *cond,
+                         og_cw_stats *st)
 {
     static const int stride[3] = {1, 16, 256}, o1[3] = {16, 1, 1}, o2[3] = {256, 256, 16};
     double *LE = (double *)calloc(OG_CUBE, sizeof(double));
@@ -478,9 +492,11 @@ static int og_decode_one(const double *Lch, int L, int Imax, uint64_t Tmax, doub
                     if (og_siso(v, L, Tmax, thres, la, le, &tr)) rc = -1;
                     ng += tr.T;
                     if (tr.T > gmax) gmax = tr.T;
+                    const double kc = og_chain_cond(&tr);
                     for (int j = 0; j < 16; j++) {
                         LA[base + j * stride[ax]] = la[j];
                         LE[base + j * stride[ax]] = le[j];
+                        if (cond) cond[base + j * stride[ax]] = kc;
                     }
                 }
             ngp += gmax;
@@ -521,6 +537,7 @@ static void *og_worker(void *arg)
         int rc = og_decode_one(J->llr + c * OG_CUBE, J->L, J->Imax, J->Tmax, J->thres, J->alpha,
                                J->bits ? J->bits + c * 64 : NULL,
                                J->soft ? J->soft + c * OG_CUBE : NULL,
+                               J->cond ? J->cond + c * OG_CUBE : NULL,
                                J->stats ? J->stats + c : NULL);
         if (rc) {
             pthread_mutex_lock(&J->mu);
@@ -534,6 +551,14 @@ static void *og_worker(void *arg)
 int og_decode_cubes(const double *llr, size_t n_cw, int L, int Imax, uint64_t Tmax, double p_et,
                     double alpha, uint8_t *bits, double *soft, og_cw_stats *stats, int threads)
 {
+    return og_decode_cubes_cond(llr, n_cw, L, Imax, Tmax, p_et, alpha, bits, soft, NULL, stats,
+                                threads);
+}
+
+int og_decode_cubes_cond(const double *llr, size_t n_cw, int L, int Imax, uint64_t Tmax,
+                         double p_et, double alpha, uint8_t *bits, double *soft, double *cond,
+                         og_cw_stats *stats, int threads)
+{
     og_init();
     og_job J;
     J.llr = llr;
@@ -545,6 +570,7 @@ int og_decode_cubes(const double *llr, size_t n_cw, int L, int Imax, uint64_t Tm
     J.alpha = alpha;
     J.bits = bits;
     J.soft = soft;
+    J.cond = cond;
     J.stats = stats;
     J.next = 0;
     J.bad = 0;

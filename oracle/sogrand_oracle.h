@@ -56,7 +56,7 @@ void og_sort_with_perm(double *a, uint8_t *perm, int n);
  * how many were written (the schedule holds 65535 nonzero patterns). */
 int og_schedule_prefix(int32_t ic, uint16_t *masks, int count);
 
-/* Component SISO decode of one line. Returns 0, or -1 on a NaN input.
+/* Component SISO decode of one line. Returns 0, or -1 on a non-finite input.
  * L_APP / L_E may be NULL; trace may be NULL. */
 int og_siso(const double *llr, int L, uint64_t Tmax, double thres, double *L_APP, double *L_E,
             og_line_trace *trace);
@@ -67,9 +67,17 @@ int og_cube_is_codeword(const double *L_APP);
 /* Decode n_cw cubes (4096 f64 each, index a0 + 16*a1 + 256*a2).
  * bits: 64 B per codeword, MSB-first packed info bits b = a0 + 8*a1 + 64*a2.
  * soft (optional): final L_APP, 4096 f64 per codeword. stats (optional).
- * threads <= 0 means all online CPUs. Returns 0, or -1 if any input is NaN. */
+ * threads <= 0 means all online CPUs. Returns 0, or -1 if any input is not finite. */
 int og_decode_cubes(const double *llr, size_t n_cw, int L, int Imax, uint64_t Tmax, double p_et,
                     double alpha, uint8_t *bits, double *soft, og_cw_stats *stats, int threads);
+
+/* As og_decode_cubes, plus cond (optional, 4096 per codeword): for each final
+ * L_APP value, the relative rounding-error bound of P_notGuess in the line
+ * decode that produced it (the reference's own arithmetic determines that
+ * L_APP only to about this many units). */
+int og_decode_cubes_cond(const double *llr, size_t n_cw, int L, int Imax, uint64_t Tmax,
+                         double p_et, double alpha, uint8_t *bits, double *soft, double *cond,
+                         og_cw_stats *stats, int threads);
 
 #ifdef __cplusplus
 }

@@ -45,7 +45,7 @@ EXPORTED = [
     "sogrand_params_default", "sogrand_strerror", "sogrand_ctx_create", "sogrand_ctx_set_params",
     "sogrand_ctx_destroy", "sogrand_decode_cubic", "sogrand_bitso_lines", "sogrand_last_kernel_ms",
     "sogrand_sigma_for_ebn0", "sogrand_gen_info_bits", "sogrand_encode_cubes", "sogrand_gen_llr",
-    "sogrand_gen_llr_device",
+    "sogrand_gen_llr_device", "sogrand_schedule_table",
 ]
 
 
@@ -67,6 +67,8 @@ class Params(C.Structure):
         ("backend", C.c_int32),
         ("wave_slots", C.c_int32),
         ("waves_per_cu", C.c_int32),
+        ("lane_batch", C.c_int32),
+        ("step_len", C.c_int32),
     ]
 
 
@@ -102,6 +104,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.sogrand_gen_llr.argtypes = [C.c_uint64, C.c_uint64, C.c_size_t, vp, C.c_size_t, vp]
     lib.sogrand_gen_llr_device.argtypes = [vp, C.c_uint64, C.c_uint64, C.c_size_t, vp, C.c_size_t,
                                            vp, vp]
+    lib.sogrand_schedule_table.argtypes = [C.c_int32, vp]
     _lib = lib
     return lib
 
@@ -230,6 +233,13 @@ class Decoder:
 
 
 # ---- host-side data helpers (no GPU needed) ----
+
+def schedule_table(ic: int) -> np.ndarray:
+    """The 65535-pattern schedule for intercept ic (plus the 0 end marker)."""
+    out = np.zeros(65536, np.uint16)
+    _check(load_library().sogrand_schedule_table(ic, out.ctypes.data), "schedule_table")
+    return out
+
 
 def sigma_for_ebn0(ebn0_db: float) -> float:
     return load_library().sogrand_sigma_for_ebn0(ebn0_db)

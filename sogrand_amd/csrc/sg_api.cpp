@@ -90,6 +90,8 @@ void sogrand_params_default(sogrand_params *p)
     p->backend = SOGRAND_BACKEND_PERSISTENT;
     p->wave_slots = 0;
     p->waves_per_cu = 0;
+    p->lane_batch = 0;
+    p->step_len = 0;
 }
 
 const char *sogrand_strerror(int code)
@@ -117,7 +119,9 @@ static int check_params(const sogrand_params *p)
     if (p->precision != SOGRAND_FP64 && p->precision != SOGRAND_FP32_MIXED) return SOGRAND_EINVAL;
     if (p->backend != SOGRAND_BACKEND_PLAIN && p->backend != SOGRAND_BACKEND_PERSISTENT)
         return SOGRAND_EINVAL;
-    if (p->wave_slots < 0 || p->wave_slots > 8 || p->waves_per_cu < 0 || p->waves_per_cu > 32)
+    if (p->wave_slots < 0 || p->wave_slots > 2 || p->waves_per_cu < 0 || p->waves_per_cu > 32)
+        return SOGRAND_EINVAL;
+    if (p->lane_batch < 0 || p->lane_batch > 64 || p->step_len < 0 || p->step_len > 65536)
         return SOGRAND_EINVAL;
     return SOGRAND_OK;
 }
@@ -231,9 +235,13 @@ static int decode_device_chunk(sogrand_ctx *ctx, const double *d_llr, size_t n, 
     hipError_t e;
     if (p.backend == SOGRAND_BACKEND_PERSISTENT) {
         const int slots = p.wave_slots > 0 ? p.wave_slots : 2;
-        const int wpc = p.waves_per_cu > 0 ? p.waves_per_cu : 12;
-        const int wpb = 4; // waves per workgroup of the persistent kernel
-        int blocks = (ctx->cu_count * wpc + wpb - 1) / wpb;
+        const int wpb = sg::kPersistentWaves;
+        // a persistent grid must be resident all at once: never more workgroups
+        // per CU than the occupancy the kernel's registers and LDS allow
+        const int occ = sg::persistent_blocks_per_cu(slots, job.tmax < (1ull << 17));
+        int bpc = p.waves_per_cu > 0 ? (p.waves_per_cu + wpb - 1) / wpb : occ;
+        if (bpc > occ) bpc = occ;
+        int blocks = ctx->cu_count * bpc;
         // never more workers than cubes
         const size_t max_blocks = (n * 1 + (size_t)wpb * slots - 1) / ((size_t)wpb * slots) + 1;
         if ((size_t)blocks > max_blocks) blocks = (int)max_blocks;
@@ -244,7 +252,9 @@ static int decode_device_chunk(sogrand_ctx *ctx, const double *d_llr, size_t n, 
         job.queue = ctx->queue.p;
         if (sg::launch_fill_u32(ctx->queue.p, 0u, 4, st) != hipSuccess) return SOGRAND_EHIP;
         (void)hipEventRecord(ctx->ev0, st);
-        e = sg::launch_persistent(job, wpb, blocks, slots, st);
+        const uint32_t batch = p.lane_batch > 0 ? (uint32_t)p.lane_batch : 8u;
+        const uint32_t step = p.step_len > 0 ? (uint32_t)p.step_len : 64u;
+        e = sg::launch_persistent(job, wpb, blocks, slots, batch, step, st);
     } else {
         const size_t slots = sg::plain_scratch_slots(n);
         if (ctx->le_scratch.reserve(slots * sg::kCube) != hipSuccess) return SOGRAND_ENOMEM;
@@ -386,6 +396,13 @@ int sogrand_last_kernel_ms(sogrand_ctx *ctx, double *ms, uint32_t *launches)
     if (!ctx) return SOGRAND_EINVAL;
     if (ms) *ms = ctx->last_ms;
     if (launches) *launches = ctx->last_launches;
+    return SOGRAND_OK;
+}
+
+int sogrand_schedule_table(int32_t ic, uint16_t *out)
+{
+    if (!out || ic < 0) return SOGRAND_EINVAL;
+    sg::build_schedule_table(ic < sg::kSchedGenericIc ? ic : sg::kSchedGenericIc, out);
     return SOGRAND_OK;
 }
 

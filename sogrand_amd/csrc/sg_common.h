@@ -19,7 +19,7 @@ constexpr int kN = 16;         // code length per dimension
 constexpr int kK = 8;          // info bits per dimension
 constexpr int kCube = 4096;    // LLRs per codeword
 constexpr int kInfoBytes = 64; // packed info bits per codeword
-constexpr int kMaxL = 4;       // largest list size the kernels hold in registers
+constexpr int kMaxL = 3;       // largest list size the kernels hold in registers
 
 // ---------------------------------------------------------------------------
 // (16,8) systematic CRC code, generator polynomial from Koopman 0xeb:

@@ -16,6 +16,7 @@
 #include <stdint.h>
 
 #include "sg_common.h"
+#include "sg_exp_table.h"
 
 #pragma clang fp contract(off)
 
@@ -26,7 +27,42 @@ struct DecodeCfg {
     double thres;          // 1 - p_ET
     const uint16_t *sched; // kSchedTables x kSchedStride masks
     uint32_t L;            // list size 1..kMaxL
+    const ExpPair *exptab; // kExp2Table copied to LDS by the kernel
 };
+
+// Copy the exp table into LDS (first 32 threads) - the caller synchronises.
+__device__ __forceinline__ void load_exp_table(ExpPair *lds, int tid)
+{
+    if (tid < 32) lds[tid] = kExp2Table[tid];
+}
+
+// exp(-pm) for pm >= 0, the probability weight of one guess (:626, :672, :712).
+// Tang-style: -pm = (32k + j) ln2/32 + r with |r| <= ln2/64, exp(r) - 1 = q by
+// its Taylor polynomial to r^6, and 2^(j/32) = hi + lo from the LDS table; the
+// result hi + (hi q + lo (1 + q)) scaled by 2^k. Within ~0.5 ulp (it differs
+// from the correctly rounded value on about 1 input in 200). Underflows to 0
+// beyond pm = 745.5, as exp does.
+__device__ __forceinline__ double exp_neg(double pm, const ExpPair *tab)
+{
+    const bool under = !(pm < 745.5);
+    const double x = under ? -745.5 : -pm;
+    const double kd = __builtin_rint(x * kExpInvLn2N);
+    double r = __builtin_fma(-kd, kExpLn2NHi, x);
+    r = __builtin_fma(-kd, kExpLn2NLo, r);
+    double q = 1.0 / 720.0;
+    q = __builtin_fma(q, r, 1.0 / 120.0);
+    q = __builtin_fma(q, r, 1.0 / 24.0);
+    q = __builtin_fma(q, r, 1.0 / 6.0);
+    q = __builtin_fma(q, r, 0.5);
+    q = __builtin_fma(q, r, 1.0);
+    q = q * r;
+    const int k = (int)kd;
+    const ExpPair t = tab[k & 31];
+    const double s = __builtin_fma(t.lo, q, t.lo);
+    const double m = __builtin_fma(t.hi, q, s) + t.hi;
+    const double v = __builtin_ldexp(m, k >> 5);
+    return under ? 0.0 : v;
+}
 
 // Per-lane LDS record arrays: delta (f64) and (H column | natural position << 8).
 struct LineLds {
@@ -111,11 +147,45 @@ __device__ __forceinline__ void list_put(LineList &ls, double e, uint32_t cw)
         }
 }
 
+// Set-up of a line whose magnitudes contain a tie, from d(i) = |x_i| in natural
+// order: the reference's quicksort decides the order among equal magnitudes
+// (replayed by quicksort_lds), then PM_HD in natural order and the sorted
+// records. Returns PM_HD.
+__device__ __forceinline__ double setup_with_ties(const LineLds S)
+{
+#pragma unroll 1
+    for (int i = 0; i < 16; ++i) S.h(i) = (uint16_t)(hcol_of((unsigned)i) | (i << 8));
+    quicksort_lds(S);
+    // natural-order PM_HD (:617): position i now sits at its sorted slot
+    uint64_t slot_of = 0;
+#pragma unroll 1
+    for (int b = 0; b < 16; ++b) slot_of |= (uint64_t)b << (4 * (S.h(b) >> 8));
+    double pm_hd = 0.0;
+#pragma unroll 1
+    for (int i = 0; i < 16; ++i) pm_hd += jaclog(-S.d((int)((slot_of >> (4 * i)) & 15u)));
+#pragma unroll 1
+    for (int b = 0; b < 16; ++b) {
+        const double ab = S.d(b);
+        S.d(b) = jaclog(ab) - jaclog(-ab);
+    }
+    return pm_hd;
+}
+
 // Line set-up (sogrand_main_logic :598-622): hard decision, PM of the hard
 // decision in natural order, sort, intercept, and the sorted LDS records.
-// Only the sort network holds the 16 magnitudes in registers; the
-// transcendental passes are rolled loops over the slot's LDS records, which
-// keeps the register footprint of the set-up below that of the search.
+//
+// Everything runs as rolled loops over the slot's 16 LDS records, so the set-up
+// holds only a handful of registers (it runs while other lanes of the wave keep
+// their search state live):
+//   1. gather |x_i| into d(i);
+//   2. rank by counting: rank_i = #{j : a_j < a_i} + #{j < i : a_j == a_i};
+//      any equal pair is a tie;
+//   3. without ties (the usual case) the ranks are the unique ascending order:
+//      delta_i = JacLog(a_i) - JacLog(-a_i) is formed in natural order (sharing
+//      JacLog(-a_i) with PM_HD) and moved to d(rank_i) by following the cycles
+//      of the permutation in place;
+//   4. with ties, the reference's quicksort decides the order among equal
+//      magnitudes, so it is replayed on the records (quicksort_lds).
 // Returns false if an input is not finite.
 template <class Src>
 __device__ __forceinline__ bool line_setup(const Src &src, const LineLds &S, uint32_t &hd,
@@ -124,8 +194,6 @@ __device__ __forceinline__ bool line_setup(const Src &src, const LineLds &S, uin
     bool finite = true;
     hd = 0;
     s0 = 0;
-    // gather in a rolled loop (magnitudes staged in the slot's LDS records):
-    // an unrolled gather would hold all 16 loads and their addresses at once
 #pragma unroll 1
     for (int i = 0; i < 16; ++i) {
         const double x = src(i);
@@ -137,55 +205,64 @@ __device__ __forceinline__ bool line_setup(const Src &src, const LineLds &S, uin
         S.d(i) = fabs(x);
     }
     if (!finite) return false;
-    // getPM_HD: sum of JacLog(-|llr_i|) in natural order, before the sort (:617)
-    pm_hd = 0.0;
-#pragma unroll 1
-    for (int i = 0; i < 16; ++i) pm_hd += jaclog(-S.d(i));
-    double a[16];
-    uint32_t idx[16];
-#pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        a[i] = S.d(i);
-        idx[i] = (uint32_t)hcol_of(i) | ((uint32_t)i << 8);
-    }
-    // 63-comparator network; exact for distinct keys
-#pragma unroll
-    for (int c = 0; c < kSortNet.n; ++c) {
-        const int lo = kSortNet.lo[c], hi = kSortNet.hi[c];
-        const bool sw = a[lo] > a[hi];
-        const double al = sw ? a[hi] : a[lo], ah = sw ? a[lo] : a[hi];
-        const uint32_t il = sw ? idx[hi] : idx[lo], ih = sw ? idx[lo] : idx[hi];
-        a[lo] = al;
-        a[hi] = ah;
-        idx[lo] = il;
-        idx[hi] = ih;
-    }
+
+    uint64_t rank = 0; // 4 bits per natural position
     bool tie = false;
-#pragma unroll
-    for (int b = 0; b < 15; ++b) tie = tie || (a[b] == a[b + 1]);
+    double a_first = 0.0, a_eighth = 0.0;
+#pragma unroll 1
+    for (int i = 0; i < 16; ++i) {
+        const double ai = S.d(i);
+        uint32_t r = 0;
+#pragma unroll 4
+        for (int j = 0; j < 16; ++j) {
+            const double aj = S.d(j);
+            r += (aj < ai || (aj == ai && j < i)) ? 1u : 0u;
+            tie = tie || (aj == ai && j != i);
+        }
+        rank |= (uint64_t)r << (4 * i);
+        if (r == 0) a_first = ai;
+        if (r == 7) a_eighth = ai;
+    }
     // IC = max(round(absL[0]/beta - 1), 0), beta = (absL[7]-absL[0])/7 (:619-622);
     // the two order statistics do not depend on how ties are arranged
-    const double beta = (a[7] - a[0]) / 7.0;
-    const int32_t r = cvtt_i32(round(a[0] / beta - 1.0));
-    ic = r > 0 ? r : 0;
+    const double beta = (a_eighth - a_first) / 7.0;
+    const int32_t rr = cvtt_i32(round(a_first / beta - 1.0));
+    ic = rr > 0 ? rr : 0;
+
+    pm_hd = 0.0;
     if (!tie) {
-#pragma unroll
-        for (int b = 0; b < 16; ++b) {
-            S.d(b) = a[b];
-            S.h(b) = (uint16_t)idx[b];
+        // getPM_HD (:617) in natural order; delta_i (getPM's increment, :765)
+#pragma unroll 1
+        for (int i = 0; i < 16; ++i) {
+            const double ai = S.d(i);
+            const double jn = jaclog(-ai);
+            pm_hd += jn;
+            S.d(i) = jaclog(ai) - jn;
+        }
+        // move delta_i to sorted position rank_i, one permutation cycle at a time
+        uint32_t placed = 0;
+#pragma unroll 1
+        for (int start = 0; start < 16; ++start) {
+            if ((placed >> start) & 1u) continue;
+            double carry = S.d(start);
+            int from = start;
+#pragma unroll 1
+            for (;;) {
+                const int to = (int)((rank >> (4 * from)) & 15u);
+                placed |= 1u << to;
+                S.h(to) = (uint16_t)(hcol_of((unsigned)from) | (from << 8));
+                if (to == start) {
+                    S.d(to) = carry;
+                    break;
+                }
+                const double t = S.d(to);
+                S.d(to) = carry;
+                carry = t;
+                from = to;
+            }
         }
     } else {
-        // equal magnitudes: reproduce the reference's quicksort tie order;
-        // S.d still holds the natural-order magnitudes
-#pragma unroll
-        for (int i = 0; i < 16; ++i) S.h(i) = (uint16_t)(hcol_of(i) | (i << 8));
-        quicksort_lds(S);
-    }
-    // delta_b = JacLog(a_b) - JacLog(-a_b): the increment getPM adds (:765)
-#pragma unroll 1
-    for (int b = 0; b < 16; ++b) {
-        const double ab = S.d(b);
-        S.d(b) = jaclog(ab) - jaclog(-ab);
+        pm_hd = setup_with_ties(S);
     }
     return true;
 }
@@ -231,7 +308,7 @@ __device__ __forceinline__ void search_init(SearchState &ss, LineList &ls, uint3
         ls.cw[q] = 0;
     }
     ss.T = 1;
-    const double e_hd = exp(-pm_hd);
+    const double e_hd = exp_neg(pm_hd, cfg.exptab);
     ss.pnot = 1.0 - e_hd;
     ss.done = false;
     if (s0 == 0) ss.done = list_hit(ls, e_hd, hd, ss.pnot, cfg);
@@ -243,54 +320,88 @@ __device__ __forceinline__ void search_init(SearchState &ss, LineList &ls, uint3
     ss.cur_wt = -1;
 }
 
-// Up to max_steps further guesses of the schedule (:652-744), streamed 8 masks
-// per 16-byte load with the next load issued one chunk ahead.
+// Up to max_steps further guesses of the schedule (:652-744), two patterns per
+// iteration so that each lane carries two independent path-metric and exp
+// chains; the guesses are still accounted strictly in schedule order (the
+// second is dropped if the first ends the search). Masks stream 8 per 16-byte
+// load with the next load issued one chunk ahead.
 template <bool FINITE_T>
 __device__ __forceinline__ void search_run(SearchState &ss, LineList &ls, const LineLds &S,
                                            uint32_t hd, uint32_t s0, double pm_hd, int32_t ic,
                                            const DecodeCfg &cfg, uint32_t max_steps)
 {
 #pragma unroll 1
-    for (uint32_t step = 0; step < max_steps && !ss.done; ++step) {
-        const uint32_t t = ss.t;
+    for (uint32_t step = 0; step < max_steps && !ss.done; step += 2) {
+        const uint32_t t = ss.t; // always even here
         if ((t & 7u) == 0) {
             ss.clo = (uint64_t)ss.nxt.x | ((uint64_t)ss.nxt.y << 32);
             ss.chi = (uint64_t)ss.nxt.z | ((uint64_t)ss.nxt.w << 32);
             ss.nxt = ss.tp[(t >> 3) + 1]; // the table buffer carries one chunk of padding
         }
         const uint32_t k = t & 7u;
-        const uint32_t m = (uint32_t)(((k < 4u) ? ss.clo : ss.chi) >> ((k & 3u) * 16u)) & 0xffffu;
-        if (m == 0) { // all 65535 patterns tested (wt > wt_max)
+        const uint64_t wd = (k < 4u) ? ss.clo : ss.chi;
+        const uint32_t sh = (k & 3u) * 16u;
+        const uint32_t m1 = (uint32_t)(wd >> sh) & 0xffffu;
+        const uint32_t m2 = (uint32_t)(wd >> (sh + 16u)) & 0xffffu;
+        if (m1 == 0) { // all 65535 patterns tested (wt > wt_max)
             ss.done = true;
             break;
         }
-        if (FINITE_T) {
-            int sp = 0;
-            for (uint32_t r = m; r; r &= r - 1u) sp += __builtin_ctz(r) + 1;
-            const int64_t wt = (int64_t)ic * __builtin_popcount(m) + sp;
-            if (wt != ss.cur_wt) { // loop guard once per weight level (:652)
+        bool take2 = m2 != 0;
+        if (FINITE_T) { // loop guard once per weight level (:652)
+            int s1 = 0, s2 = 0;
+            for (uint32_t r = m1; r; r &= r - 1u) s1 += __builtin_ctz(r) + 1;
+            for (uint32_t r = m2; r; r &= r - 1u) s2 += __builtin_ctz(r) + 1;
+            const int64_t wt1 = (int64_t)ic * __builtin_popcount(m1) + s1;
+            const int64_t wt2 = (int64_t)ic * __builtin_popcount(m2) + s2;
+            if (wt1 != ss.cur_wt) {
                 if (!((uint64_t)ss.T < cfg.tmax)) {
                     ss.done = true;
                     break;
                 }
-                ss.cur_wt = wt;
+                ss.cur_wt = wt1;
             }
+            if (take2 && wt2 != wt1 && !((uint64_t)ss.T + 1u < cfg.tmax)) take2 = false;
+            else if (take2) ss.cur_wt = wt2;
         }
+        // path metrics (ascending sorted position, as getPM adds them) and syndromes
+        double pm1 = pm_hd, pm2 = pm_hd;
+        uint32_t syn1 = s0, syn2 = s0;
+        uint32_t r1 = m1, r2 = take2 ? m2 : 0u;
+        while (r1 | r2) {
+            const int b1 = __builtin_ctz(r1 | 0x8000u), b2 = __builtin_ctz(r2 | 0x8000u);
+            const double d1 = S.d(b1), d2 = S.d(b2);
+            const uint32_t h1 = S.h(b1), h2 = S.h(b2);
+            const double p1 = pm1 + d1, p2 = pm2 + d2;
+            pm1 = r1 ? p1 : pm1;
+            pm2 = r2 ? p2 : pm2;
+            syn1 ^= r1 ? h1 : 0u;
+            syn2 ^= r2 ? h2 : 0u;
+            r1 &= r1 - 1u;
+            r2 &= r2 - 1u;
+        }
+        const double e1 = exp_neg(pm1, cfg.exptab);
+        const double e2 = exp_neg(pm2, cfg.exptab);
+        ss.T += 1;
         ss.t = t + 1;
-        double pm = pm_hd;
-        uint32_t syn = s0;
-        for (uint32_t r = m; r; r &= r - 1u) {
-            const int b = __builtin_ctz(r);
-            pm += S.d(b);
-            syn ^= S.h(b);
+        ss.pnot -= e1;
+        if ((syn1 & 0xffu) == 0) {
+            uint32_t cw = hd;
+            for (uint32_t r = m1; r; r &= r - 1u) cw ^= 1u << (S.h(__builtin_ctz(r)) >> 8);
+            ss.done = list_hit(ls, e1, cw, ss.pnot, cfg);
+        }
+        if (ss.done) break;
+        if (!take2) { // end of the schedule, or the next weight level is over budget
+            ss.done = true;
+            break;
         }
         ss.T += 1;
-        const double e = exp(-pm);
-        ss.pnot -= e;
-        if ((syn & 0xffu) == 0) {
+        ss.t = t + 2;
+        ss.pnot -= e2;
+        if ((syn2 & 0xffu) == 0) {
             uint32_t cw = hd;
-            for (uint32_t r = m; r; r &= r - 1u) cw ^= 1u << (S.h(__builtin_ctz(r)) >> 8);
-            ss.done = list_hit(ls, e, cw, ss.pnot, cfg);
+            for (uint32_t r = m2; r; r &= r - 1u) cw ^= 1u << (S.h(__builtin_ctz(r)) >> 8);
+            ss.done = list_hit(ls, e2, cw, ss.pnot, cfg);
         }
     }
 }

@@ -84,10 +84,13 @@ __global__ __launch_bounds__(kPlainThreads) void plain_kernel(CubeJob job)
     __shared__ uint16_t s_hp[16 * kPlainThreads];
     __shared__ uint8_t s_hd[kCube];
     __shared__ uint32_t s_sum[kPlainThreads / 64], s_max[kPlainThreads / 64];
+    __shared__ ExpPair s_exp[32];
 
     const int t = threadIdx.x;
+    load_exp_table(s_exp, t);
+    __syncthreads();
     const LineLds S{s_delta, s_hp, kPlainThreads, t};
-    const DecodeCfg cfg{job.tmax, job.thres, job.sched, job.L};
+    const DecodeCfg cfg{job.tmax, job.thres, job.sched, job.L, s_exp};
     double *le = job.le_scratch + (size_t)blockIdx.x * kCube;
 
     for (size_t cube = blockIdx.x; cube < job.n_cw; cube += gridDim.x) {
@@ -193,6 +196,10 @@ __global__ __launch_bounds__(256) void lines_kernel(const double *llr, size_t n,
 {
     __shared__ double s_delta[16 * 256];
     __shared__ uint16_t s_hp[16 * 256];
+    __shared__ ExpPair s_exp[32];
+    load_exp_table(s_exp, threadIdx.x);
+    __syncthreads();
+    cfg.exptab = s_exp;
     const size_t line = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (line >= n) return;
     const LineLds S{s_delta, s_hp, 256, (int)threadIdx.x};
@@ -316,7 +323,7 @@ hipError_t launch_lines(const double *llr, size_t n_lines, double *la, double *l
                         double thres, uint32_t L, hipStream_t st)
 {
     if (n_lines == 0) return hipSuccess;
-    const DecodeCfg cfg{tmax, thres, sched, L};
+    const DecodeCfg cfg{tmax, thres, sched, L, nullptr};
     const unsigned grid = (unsigned)((n_lines + 255) / 256);
     if (tmax >= (1ull << 17))
         hipLaunchKernelGGL(lines_kernel<false>, dim3(grid), dim3(256), 0, st, llr, n_lines, la, le,

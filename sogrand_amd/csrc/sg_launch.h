@@ -32,7 +32,10 @@ size_t plain_scratch_slots(size_t n_cw);
 // Persistent backend: waves pop cubes from a device counter; each wave keeps
 // `slots` cubes in flight and hands lines to lanes as they free up.
 hipError_t launch_persistent(const CubeJob &job, int waves_per_block, int blocks, int slots,
-                             hipStream_t st);
+                             uint32_t lane_batch, uint32_t step_len, hipStream_t st);
+// Resident workgroups per CU of the persistent kernel (occupancy API).
+int persistent_blocks_per_cu(int slots, bool finite);
+constexpr int kPersistentWaves = 4;
 
 // Line-level SISO (SOGRAND_bitSO on many lines).
 hipError_t launch_lines(const double *llr, size_t n_lines, double *la, double *le,

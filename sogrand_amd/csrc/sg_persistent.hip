@@ -65,24 +65,72 @@ struct PSrc {
     }
 };
 
-// Bit-sliced component check of the 16 columns of a 16x16 bit plane W
-// (bit q of W[j] = row j, column q): column q is a codeword iff all 8 syndrome
-// rows vanish; X_r = W[8+r] ^ XOR_{j<8, P[j][r]=1} W[j].
-__device__ __forceinline__ bool plane_cols_ok(const uint32_t (&W)[16])
+// Early-termination test of one 16x16 bit plane streamed word by word: word j
+// (bit q = column q) is checked as a component codeword itself when `rows` is
+// set, and folded into the 8 column syndromes X_r = W[8+r] ^ XOR_{j<8, P[j][r]}
+// W[j], which vanish iff every column q is a component codeword.
+template <class Word>
+__device__ __forceinline__ bool plane_ok(const Word &word, bool rows)
 {
+    uint32_t x[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    bool ok = true;
+#pragma unroll 1
+    for (int j = 0; j < 16; ++j) {
+        const uint32_t w = word(j);
+        if (rows) ok = ok && (syndrome16(w) == 0);
+        const uint32_t h = hcol_of((unsigned)j); // column j of H: rows r it feeds
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+            if ((h >> r) & 1u) x[r] ^= w;
+    }
     uint32_t bad = 0;
 #pragma unroll
-    for (int r = 0; r < 8; ++r) {
-        uint32_t x = W[8 + r];
-#pragma unroll
-        for (int j = 0; j < 8; ++j)
-            if ((hcol(j) >> r) & 1u) x ^= W[j];
-        bad |= x;
-    }
-    return (bad & 0xffffu) == 0;
+    for (int r = 0; r < 8; ++r) bad |= x[r];
+    return ok && (bad & 0xffffu) == 0;
 }
 
 } // namespace
+
+// Lane states of the persistent kernel.
+enum : int { kIdle = 0, kSearch = 1, kDone = 2, kNew = 3 };
+
+template <int S> struct SlotState {
+    uint32_t cube[S], it[S], ax[S], nxt_line[S], half[S];
+    uint64_t ng[S], ngp[S];
+    bool active[S];
+    // per-lane selection of a slot's value (s is per lane; the arrays are uniform)
+    __device__ uint32_t sel(const uint32_t (&v)[S], int s) const
+    {
+        uint32_t r = v[0];
+#pragma unroll
+        for (int k = 1; k < S; ++k)
+            if (s == k) r = v[k];
+        return r;
+    }
+};
+
+// The lowest four set lanes of a wave mask (owners of one transition sub-round),
+// -1 where there are fewer.
+__device__ __forceinline__ void take4(uint64_t &m, int (&o)[4])
+{
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        if (m) {
+            o[q] = __builtin_ctzll(m);
+            m &= m - 1ull;
+        } else {
+            o[q] = -1;
+        }
+    }
+}
+
+__device__ __forceinline__ double shfl_d(double v, int src)
+{
+    const uint64_t u = double_to_bits(v);
+    const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)u, src, 64);
+    const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(u >> 32), src, 64);
+    return bits_to_double(((uint64_t)hi << 32) | lo);
+}
 
 template <bool FINITE_T, int S>
 __global__ __launch_bounds__(64 * kWaves) void persistent_kernel(CubeJob job, uint32_t thresh,
@@ -92,98 +140,110 @@ __global__ __launch_bounds__(64 * kWaves) void persistent_kernel(CubeJob job, ui
     __shared__ uint16_t s_h[kWaves][16 * 64];
     __shared__ uint32_t s_hd[kWaves][S][128];
     __shared__ uint32_t s_cnt[kWaves][S][4]; // lines done, phase guess sum, phase max, bad
+    __shared__ double s_stage_jn[kWaves][64];
+    __shared__ uint32_t s_stage_rk[kWaves][64];
+    __shared__ ExpPair s_exp[32];
 
+    load_exp_table(s_exp, threadIdx.x);
+    __syncthreads(); // the only workgroup barrier: waves run independently from here
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint64_t lt_mask = (1ull << lane) - 1ull;
-    const LineLds L{&s_d[wave][0], &s_h[wave][0], 64, lane};
-    const DecodeCfg cfg{job.tmax, job.thres, job.sched, job.L};
+    double *rec_d = &s_d[wave][0];
+    uint16_t *rec_h = &s_h[wave][0];
+    const LineLds L{rec_d, rec_h, 64, lane};
+    const DecodeCfg cfg{job.tmax, job.thres, job.sched, job.L, s_exp};
     const size_t wave_id = (size_t)blockIdx.x * kWaves + wave;
     uint32_t *hd_bits = &s_hd[wave][0][0];
     uint32_t *cnt = &s_cnt[wave][0][0];
+    double *stage_jn = &s_stage_jn[wave][0];
+    uint32_t *stage_rk = &s_stage_rk[wave][0];
 
-    // wave-uniform slot state
-    uint32_t cube[S], it[S], ax[S], nxt_line[S], half[S];
-    uint64_t ng[S], ngp[S];
-    bool active[S];
+    SlotState<S> sl;
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-        active[s] = false;
-        cube[s] = it[s] = ax[s] = nxt_line[s] = half[s] = 0;
-        ng[s] = ngp[s] = 0;
+        sl.active[s] = false;
+        sl.cube[s] = sl.it[s] = sl.ax[s] = sl.nxt_line[s] = sl.half[s] = 0;
+        sl.ng[s] = sl.ngp[s] = 0;
     }
     bool queue_empty = false;
 
-    // lane state
-    int my_slot = -1;
-    const double *lch = nullptr;
-    double *le = nullptr, *soft = nullptr;
-    int base = 0, stride = 1;
-    bool first = false;
+    // lane state: the line this lane owns and its search
+    int state = kIdle, my_slot = 0, my_line = 0;
     uint32_t hd = 0, s0 = 0;
     double pm_hd = 0.0;
     int32_t ic = 0;
     LineList ls;
     SearchState ss;
+    ls.nl = 0;
+    ls.pnl = 0.0;
+    ss.T = 0;
     ss.done = true;
+
+    // geometry of (slot, line) as seen by any lane
+    auto line_geom = [&](int s, int line, int &base, int &stride, const double *&lch, double *&le,
+                         double *&soft, bool &first) {
+        const uint32_t ax = sl.sel(sl.ax, s), c = sl.sel(sl.cube, s), it = sl.sel(sl.it, s);
+        axis_geom((int)ax, line, base, stride);
+        lch = job.llr + (size_t)c * kCube;
+        le = job.le_scratch + (wave_id * S + (size_t)s) * kCube;
+        soft = job.soft ? job.soft + (size_t)c * kCube : nullptr;
+        first = (it == 1 && ax == 0);
+    };
 
     for (;;) {
         // ---- 1. phase completion / cube retirement (wave-uniform) ----
 #pragma unroll
         for (int s = 0; s < S; ++s) {
-            if (!active[s] || nxt_line[s] < 256) continue;
+            if (!sl.active[s] || sl.nxt_line[s] < 256) continue;
             const uint32_t done_lines = __builtin_amdgcn_readfirstlane(cnt[4 * s + 0]);
             if (done_lines < 256) continue;
             // all lines of the phase are in: make their L_E stores visible to
             // the next phase's loads
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-            ng[s] += __builtin_amdgcn_readfirstlane(cnt[4 * s + 1]);
-            ngp[s] += __builtin_amdgcn_readfirstlane(cnt[4 * s + 2]);
+            sl.ng[s] += __builtin_amdgcn_readfirstlane(cnt[4 * s + 1]);
+            sl.ngp[s] += __builtin_amdgcn_readfirstlane(cnt[4 * s + 2]);
             // early termination: all 768 lines of the hard decision are
             // component codewords (equivalent to the re-encode test :317-391)
             bool ok = true;
             const uint32_t *hb = hd_bits + 128 * s;
             if (lane < 32) {
-                uint32_t W[16];
+                // lanes 0-15: plane a2 = q (words over a1: axis-0 lines, columns =
+                // axis-1 lines); lanes 16-31: plane a1 = q (columns = axis-2 lines)
                 const int q = lane & 15;
-#pragma unroll
-                for (int j = 0; j < 16; ++j) {
-                    // lanes 0-15: plane a2 = q, words over a1; lanes 16-31: plane a1 = q, words over a2
-                    const int h = lane < 16 ? (j + 16 * q) : (q + 16 * j);
-                    W[j] = (hb[h >> 1] >> (16 * (h & 1))) & 0xffffu;
-                }
-                ok = plane_cols_ok(W);
-                if (lane < 16) {
-#pragma unroll
-                    for (int j = 0; j < 16; ++j) ok = ok && (syndrome16(W[j]) == 0);
-                }
+                const bool a2_plane = lane < 16;
+                auto word = [&](int j) {
+                    const int h = a2_plane ? (j + 16 * q) : (q + 16 * j);
+                    return (hb[h >> 1] >> (16 * (h & 1))) & 0xffffu;
+                };
+                ok = plane_ok(word, a2_plane);
             }
             const bool conv = __ballot(!ok) == 0ull;
-            const bool last = (ax[s] == 2 && it[s] >= job.Imax);
+            const bool last = (sl.ax[s] == 2 && sl.it[s] >= job.Imax);
             if (conv || last) {
                 // retire: info bits from the hard decision (:267-280), statistics
                 const int h = (lane & 7) + 16 * (lane >> 3);
                 const uint32_t byte = rev8((hb[h >> 1] >> (16 * (h & 1))) & 0xffu);
-                job.info[(size_t)cube[s] * kInfoBytes + lane] = (uint8_t)byte;
+                job.info[(size_t)sl.cube[s] * kInfoBytes + lane] = (uint8_t)byte;
                 if (lane == 0) {
                     if (job.stats) {
-                        CwStatsOut *o = reinterpret_cast<CwStatsOut *>(job.stats) + cube[s];
-                        o->half_iters = half[s];
+                        CwStatsOut *o = reinterpret_cast<CwStatsOut *>(job.stats) + sl.cube[s];
+                        o->half_iters = sl.half[s];
                         o->converged = conv ? 1u : 0u;
-                        o->NG = ng[s];
-                        o->NG_p = ngp[s];
+                        o->NG = sl.ng[s];
+                        o->NG_p = sl.ngp[s];
                     }
-                    job.status[cube[s]] = cnt[4 * s + 3] ? 1 : 0;
+                    job.status[sl.cube[s]] = cnt[4 * s + 3] ? 1 : 0;
                 }
-                active[s] = false;
+                sl.active[s] = false;
             } else {
-                if (ax[s] == 2) {
-                    ax[s] = 0;
-                    it[s] += 1;
+                if (sl.ax[s] == 2) {
+                    sl.ax[s] = 0;
+                    sl.it[s] += 1;
                 } else {
-                    ax[s] += 1;
+                    sl.ax[s] += 1;
                 }
-                half[s] += 1; // n_iter += 0.5 before each phase (:145, :184, :223)
-                nxt_line[s] = 0;
+                sl.half[s] += 1; // n_iter += 0.5 before each phase (:145, :184, :223)
+                sl.nxt_line[s] = 0;
                 hd_bits[128 * s + lane] = 0;
                 hd_bits[128 * s + 64 + lane] = 0;
                 if (lane < 3) cnt[4 * s + lane] = 0;
@@ -192,7 +252,7 @@ __global__ __launch_bounds__(64 * kWaves) void persistent_kernel(CubeJob job, ui
         // ---- 2. refill empty slots from the device queue ----
 #pragma unroll
         for (int s = 0; s < S; ++s) {
-            if (active[s] || queue_empty) continue;
+            if (sl.active[s] || queue_empty) continue;
             uint32_t c = 0;
             if (lane == 0) c = atomicAdd(job.queue, 1u);
             c = __builtin_amdgcn_readfirstlane(c);
@@ -200,99 +260,226 @@ __global__ __launch_bounds__(64 * kWaves) void persistent_kernel(CubeJob job, ui
                 queue_empty = true;
                 continue;
             }
-            active[s] = true;
-            cube[s] = c;
-            it[s] = 1;
-            ax[s] = 0;
-            half[s] = 1;
-            nxt_line[s] = 0;
-            ng[s] = ngp[s] = 0;
+            sl.active[s] = true;
+            sl.cube[s] = c;
+            sl.it[s] = 1;
+            sl.ax[s] = 0;
+            sl.half[s] = 1;
+            sl.nxt_line[s] = 0;
+            sl.ng[s] = sl.ngp[s] = 0;
             hd_bits[128 * s + lane] = 0;
             hd_bits[128 * s + 64 + lane] = 0;
             if (lane < 4) cnt[4 * s + lane] = 0;
         }
         bool any_active = false;
+        uint32_t avail = 0;
 #pragma unroll
-        for (int s = 0; s < S; ++s) any_active = any_active || active[s];
+        for (int s = 0; s < S; ++s) {
+            any_active = any_active || sl.active[s];
+            if (sl.active[s]) avail += 256u - sl.nxt_line[s];
+        }
         if (!any_active) break;
 
-        // ---- 3. hand lines to free lanes, in batches ----
-        uint64_t idle = __ballot(my_slot < 0);
-        const uint32_t n_idle = (uint32_t)__popcll(idle);
-        bool fresh = false;
-        if (n_idle >= thresh || n_idle == 64u) {
+        // ---- 3. transitions, cooperatively, four lines per sub-round ----
+        const uint32_t n_done = (uint32_t)__popcll(__ballot(state == kDone));
+        const uint32_t n_idle = (uint32_t)__popcll(__ballot(state == kIdle));
+        const uint32_t n_search = (uint32_t)__popcll(__ballot(state == kSearch));
+        const uint32_t n_new = n_idle < avail ? n_idle : avail;
+        if (n_done + n_new >= thresh || n_search == 0) {
+            // 3a. soft output of finished lines (SOGRAND_bitSO :536-575): lane
+            // 16q + i computes position i of the q-th finished line
+            uint64_t D = __ballot(state == kDone);
+            while (D) {
+                int o[4];
+                take4(D, o);
+                const int q = lane >> 4, i = lane & 15;
+                const int owner = q == 0 ? o[0] : q == 1 ? o[1] : q == 2 ? o[2] : o[3];
+                const int src = owner >= 0 ? owner : 0;
+                const int o_slot = __shfl(my_slot, src, 64);
+                const int o_line = __shfl(my_line, src, 64);
+                LineList ol;
+                ol.nl = __shfl(ls.nl, src, 64);
+                ol.pnl = shfl_d(ls.pnl, src);
+#pragma unroll
+                for (int j = 0; j < kMaxL; ++j) {
+                    ol.e[j] = shfl_d(ls.e[j], src);
+                    ol.cw[j] = (uint32_t)__shfl((int)ls.cw[j], src, 64);
+                }
+                if (owner >= 0) {
+                    int base, stride;
+                    const double *lch;
+                    double *le, *soft;
+                    bool first;
+                    line_geom(o_slot, o_line, base, stride, lch, le, soft, first);
+                    const int p = base + i * stride;
+                    const double e = first ? 0.0 : le[p];
+                    const double tt = job.alpha * e;
+                    const double xin = lch[p] + tt;
+                    double la, lenew;
+                    if (ol.nl == 0) { // empty list: L_APP = llr, L_E = 0 (:537-539)
+                        la = xin;
+                        lenew = 0.0;
+                    } else {
+                        la = soft_app(ol, fmax(ol.pnl, 1e-40), i, xin);
+                        lenew = la - xin;
+                    }
+                    le[p] = lenew;
+                    if (soft) soft[p] = la;
+                    if (!(la > 0.0)) atomicOr(&hd_bits[128 * o_slot + (p >> 5)], 1u << (p & 31));
+                }
+                if (lane == o[0] || lane == o[1] || lane == o[2] || lane == o[3]) {
+                    atomicAdd(&cnt[4 * my_slot + 1], ss.T);
+                    atomicMax(&cnt[4 * my_slot + 2], ss.T);
+                    atomicAdd(&cnt[4 * my_slot + 0], 1u);
+                    state = kIdle;
+                }
+            }
+            // 3b. hand the next lines of the active phases to free lanes
+            uint64_t idle = __ballot(state == kIdle);
 #pragma unroll
             for (int s = 0; s < S; ++s) {
-                if (!active[s] || nxt_line[s] >= 256 || idle == 0ull) continue;
-                const uint32_t avail = 256u - nxt_line[s];
-                const uint32_t k = (uint32_t)__popcll(idle) < avail ? (uint32_t)__popcll(idle) : avail;
+                if (!sl.active[s] || sl.nxt_line[s] >= 256 || idle == 0ull) continue;
+                const uint32_t left = 256u - sl.nxt_line[s];
+                const uint32_t ni = (uint32_t)__popcll(idle);
+                const uint32_t k = ni < left ? ni : left;
                 const uint32_t rank = (uint32_t)__popcll(idle & lt_mask);
-                if (my_slot < 0 && rank < k) {
+                if (state == kIdle && rank < k) {
+                    state = kNew;
                     my_slot = s;
-                    const int line = (int)(nxt_line[s] + rank);
-                    axis_geom((int)ax[s], line, base, stride);
-                    lch = job.llr + (size_t)cube[s] * kCube;
-                    le = job.le_scratch + (wave_id * S + s) * kCube;
-                    soft = job.soft ? job.soft + (size_t)cube[s] * kCube : nullptr;
-                    first = (it[s] == 1 && ax[s] == 0);
-                    fresh = true;
+                    my_line = (int)(sl.nxt_line[s] + rank);
                 }
-                nxt_line[s] += k;
-                idle = __ballot(my_slot < 0);
+                sl.nxt_line[s] += k;
+                idle = __ballot(state == kIdle);
             }
-        }
-        // ---- 4. set-up of newly assigned lines ----
-        if (fresh) {
-            const PSrc src{lch, le, job.alpha, base, stride, first};
-            if (line_setup(src, L, hd, s0, pm_hd, ic)) {
-                search_init(ss, ls, hd, s0, pm_hd, ic, cfg);
-            } else {
-                atomicOr(&cnt[4 * my_slot + 3], 1u);
-                ss.done = true;
-                ss.T = 0;
-                ls.nl = 0;
-                ls.pnl = 0.0;
-            }
-        }
-        // ---- 5. a bounded step of every busy lane's guessing loop ----
-        if (my_slot >= 0 && !ss.done)
-            search_run<FINITE_T>(ss, ls, L, hd, s0, pm_hd, ic, cfg, step_len);
-        // ---- 6. finished lines: soft output, L_E, hard decisions, counters ----
-        if (my_slot >= 0 && ss.done) {
-            const PSrc src{lch, le, job.alpha, base, stride, first};
-            const double pnotin = fmax(ls.pnl, 1e-40);
-            uint32_t *hb = hd_bits + 128 * my_slot;
+            // 3c. set-up of the new lines (sogrand_main_logic :598-622)
+            uint64_t N = __ballot(state == kNew);
+            while (N) {
+                int o[4];
+                take4(N, o);
+                const int q = lane >> 4, i = lane & 15;
+                const int owner = q == 0 ? o[0] : q == 1 ? o[1] : q == 2 ? o[2] : o[3];
+                const int src = owner >= 0 ? owner : 0;
+                const int o_slot = __shfl(my_slot, src, 64);
+                const int o_line = __shfl(my_line, src, 64);
+                const LineLds R{rec_d, rec_h, 64, src}; // the owner's records
+                bool fin = true, tie = false;
+                double ai = 0.0, delta = 0.0;
+                uint32_t rank = 0;
+                if (owner >= 0) {
+                    int base, stride;
+                    const double *lch;
+                    double *le, *soft;
+                    bool first;
+                    line_geom(o_slot, o_line, base, stride, lch, le, soft, first);
+                    const int p = base + i * stride;
+                    const double e = first ? 0.0 : le[p];
+                    const double tt = job.alpha * e; // L_A = alpha L_E, then + L_channel (:147-155)
+                    const double x = lch[p] + tt;
+                    fin = __builtin_isfinite(x);
+                    R.d(i) = x;
+                    ai = fabs(x);
+                }
+                if (owner >= 0) {
+                    // rank by counting; ties (equal magnitudes) go to the owner
+#pragma unroll 4
+                    for (int j = 0; j < 16; ++j) {
+                        const double aj = fabs(R.d(j));
+                        rank += (aj < ai || (aj == ai && j < i)) ? 1u : 0u;
+                        tie = tie || (aj == ai && j != i);
+                    }
+                    const double jn = jaclog(-ai);
+                    delta = jaclog(ai) - jn; // getPM's increment (:765)
+                    stage_jn[lane] = jn;
+                    stage_rk[lane] = rank;
+                }
+                const uint64_t special = __ballot(owner >= 0 && (tie || !fin));
+                // owners: hard decision, PM_HD in natural order, intercept
+                const int myq = lane == o[0] ? 0 : lane == o[1] ? 1 : lane == o[2] ? 2
+                              : lane == o[3] ? 3 : -1;
+                bool line_special = false, line_bad = false;
+                if (myq >= 0) {
+                    line_special = ((special >> (16 * myq)) & 0xffffull) != 0;
+                    hd = 0;
+                    s0 = 0;
+                    pm_hd = 0.0;
+                    double a_first = 0.0, a_eighth = 0.0;
 #pragma unroll 1
-            for (int i = 0; i < 16; ++i) {
-                const int p = base + i * stride;
-                const double xin = src(i);
-                double la, lenew;
-                if (ls.nl == 0) { // empty list: L_APP = llr, L_E = 0 (:537-539)
-                    la = xin;
-                    lenew = 0.0;
-                } else {
-                    la = soft_app(ls, pnotin, i, xin);
-                    lenew = la - xin;
+                    for (int j = 0; j < 16; ++j) {
+                        const double x = L.d(j);
+                        line_bad = line_bad || !__builtin_isfinite(x);
+                        if (!(x > 0.0)) { // HardDec (:758)
+                            hd |= 1u << j;
+                            s0 ^= hcol_of((unsigned)j);
+                        }
+                        pm_hd += stage_jn[16 * myq + j]; // getPM_HD (:617)
+                        const uint32_t r = stage_rk[16 * myq + j];
+                        if (r == 0) a_first = fabs(x);
+                        if (r == 7) a_eighth = fabs(x);
+                    }
+                    // IC (:619-622)
+                    const double beta = (a_eighth - a_first) / 7.0;
+                    const int32_t rr = cvtt_i32(round(a_first / beta - 1.0));
+                    ic = rr > 0 ? rr : 0;
                 }
-                le[p] = lenew;
-                if (soft) soft[p] = la;
-                if (!(la > 0.0)) atomicOr(&hb[p >> 5], 1u << (p & 31));
+                // tasks of ordinary lines write the sorted records
+                if (owner >= 0 && ((special >> (16 * q)) & 0xffffull) == 0) {
+                    R.d((int)rank) = delta;
+                    R.h((int)rank) = (uint16_t)(hcol_of((unsigned)i) | (i << 8));
+                }
+                if (myq >= 0) {
+                    if (line_bad) {
+                        atomicOr(&cnt[4 * my_slot + 3], 1u);
+                        ls.nl = 0;
+                        ls.pnl = 0.0;
+                        ss.T = 0;
+                        ss.done = true;
+                        state = kDone;
+                    } else {
+                        if (line_special) {
+#pragma unroll 1
+                            for (int j = 0; j < 16; ++j) L.d(j) = fabs(L.d(j));
+                            pm_hd = setup_with_ties(L);
+                        }
+                        search_init(ss, ls, hd, s0, pm_hd, ic, cfg);
+                        state = ss.done ? kDone : kSearch;
+                    }
+                }
             }
-            atomicAdd(&cnt[4 * my_slot + 1], ss.T);
-            atomicMax(&cnt[4 * my_slot + 2], ss.T);
-            atomicAdd(&cnt[4 * my_slot + 0], 1u);
-            my_slot = -1;
+        }
+        // ---- 4. a bounded step of every searching lane's guessing loop ----
+        if (state == kSearch) {
+            search_run<FINITE_T>(ss, ls, L, hd, s0, pm_hd, ic, cfg, step_len);
+            if (ss.done) state = kDone;
         }
     }
 }
 
+int persistent_blocks_per_cu(int slots, bool finite)
+{
+    int nb = 0;
+    hipError_t e = hipErrorInvalidValue;
+#define SG_OCC(FT, SS)                                                                        \
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, persistent_kernel<FT, SS>, 64 * kWaves, 0)
+    if (slots == 1) {
+        if (finite) SG_OCC(true, 1); else SG_OCC(false, 1);
+    } else if (slots == 2) {
+        if (finite) SG_OCC(true, 2); else SG_OCC(false, 2);
+    }
+#undef SG_OCC
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return 1;
+    }
+    return nb > 0 ? nb : 1;
+}
+
 hipError_t launch_persistent(const CubeJob &job, int waves_per_block, int blocks, int slots,
-                             hipStream_t st)
+                             uint32_t lane_batch, uint32_t step_len, hipStream_t st)
 {
     if (job.n_cw == 0) return hipSuccess;
     if (waves_per_block != kWaves) return hipErrorInvalidValue;
     const bool finite = job.tmax < (1ull << 17);
-    const uint32_t thresh = 16, step = 64;
+    const uint32_t thresh = lane_batch, step = step_len;
     const dim3 g(blocks), b(64 * kWaves);
 #define SG_LAUNCH(FT, SS)                                                                     \
     hipLaunchKernelGGL((persistent_kernel<FT, SS>), g, b, 0, st, job, thresh, step)
@@ -300,10 +487,6 @@ hipError_t launch_persistent(const CubeJob &job, int waves_per_block, int blocks
         if (finite) SG_LAUNCH(true, 1); else SG_LAUNCH(false, 1);
     } else if (slots == 2) {
         if (finite) SG_LAUNCH(true, 2); else SG_LAUNCH(false, 2);
-    } else if (slots == 3) {
-        if (finite) SG_LAUNCH(true, 3); else SG_LAUNCH(false, 3);
-    } else if (slots == 4) {
-        if (finite) SG_LAUNCH(true, 4); else SG_LAUNCH(false, 4);
     } else {
         return hipErrorInvalidValue;
     }
