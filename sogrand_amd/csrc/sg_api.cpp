@@ -234,7 +234,7 @@ static int decode_device_chunk(sogrand_ctx *ctx, const double *d_llr, size_t n, 
     job.Imax = p.Imax;
     hipError_t e;
     if (p.backend == SOGRAND_BACKEND_PERSISTENT) {
-        const int slots = p.wave_slots > 0 ? p.wave_slots : 2;
+        const int slots = p.wave_slots > 0 ? p.wave_slots : 1;
         const int wpb = sg::kPersistentWaves;
         // a persistent grid must be resident all at once: never more workgroups
         // per CU than the occupancy the kernel's registers and LDS allow
@@ -252,7 +252,7 @@ static int decode_device_chunk(sogrand_ctx *ctx, const double *d_llr, size_t n, 
         job.queue = ctx->queue.p;
         if (sg::launch_fill_u32(ctx->queue.p, 0u, 4, st) != hipSuccess) return SOGRAND_EHIP;
         (void)hipEventRecord(ctx->ev0, st);
-        const uint32_t batch = p.lane_batch > 0 ? (uint32_t)p.lane_batch : 8u;
+        const uint32_t batch = p.lane_batch > 0 ? (uint32_t)p.lane_batch : 4u;
         const uint32_t step = p.step_len > 0 ? (uint32_t)p.step_len : 64u;
         e = sg::launch_persistent(job, wpb, blocks, slots, batch, step, st);
     } else {

@@ -64,7 +64,8 @@ __device__ __forceinline__ double exp_neg(double pm, const ExpPair *tab)
     return under ? 0.0 : v;
 }
 
-// Per-lane LDS record arrays: delta (f64) and (H column | natural position << 8).
+// Per-lane LDS record arrays: delta (f64) and (H column | natural position << 8),
+// 17 records per slot: sorted positions 0..15 and a null record 16.
 struct LineLds {
     double *delta;
     uint16_t *hp;
@@ -264,6 +265,8 @@ __device__ __forceinline__ bool line_setup(const Src &src, const LineLds &S, uin
     } else {
         pm_hd = setup_with_ties(S);
     }
+    S.d(16) = 0.0; // null record read by exhausted masks in search_run
+    S.h(16) = 0;
     return true;
 }
 
@@ -364,22 +367,25 @@ __device__ __forceinline__ void search_run(SearchState &ss, LineList &ls, const 
             if (take2 && wt2 != wt1 && !((uint64_t)ss.T + 1u < cfg.tmax)) take2 = false;
             else if (take2) ss.cur_wt = wt2;
         }
-        // path metrics (ascending sorted position, as getPM adds them) and syndromes
+        // path metrics (ascending sorted position, as getPM adds them) and
+        // syndromes, two set bits of each pattern per iteration; an exhausted
+        // mask reads the null record 16 (delta +0.0, H column 0), which leaves
+        // both sums unchanged, so no lane needs a select or a branch
         double pm1 = pm_hd, pm2 = pm_hd;
         uint32_t syn1 = s0, syn2 = s0;
         uint32_t r1 = m1, r2 = take2 ? m2 : 0u;
-        while (r1 | r2) {
-            const int b1 = __builtin_ctz(r1 | 0x8000u), b2 = __builtin_ctz(r2 | 0x8000u);
-            const double d1 = S.d(b1), d2 = S.d(b2);
-            const uint32_t h1 = S.h(b1), h2 = S.h(b2);
-            const double p1 = pm1 + d1, p2 = pm2 + d2;
-            pm1 = r1 ? p1 : pm1;
-            pm2 = r2 ? p2 : pm2;
-            syn1 ^= r1 ? h1 : 0u;
-            syn2 ^= r2 ? h2 : 0u;
-            r1 &= r1 - 1u;
-            r2 &= r2 - 1u;
-        }
+        do {
+            const uint32_t q1 = r1 & (r1 - 1u), q2 = r2 & (r2 - 1u);
+            const int b1 = __builtin_ctz(r1 | 0x10000u), c1 = __builtin_ctz(q1 | 0x10000u);
+            const int b2 = __builtin_ctz(r2 | 0x10000u), c2 = __builtin_ctz(q2 | 0x10000u);
+            const double d1 = S.d(b1), e1 = S.d(c1), d2 = S.d(b2), e2 = S.d(c2);
+            syn1 ^= (uint32_t)S.h(b1) ^ (uint32_t)S.h(c1);
+            syn2 ^= (uint32_t)S.h(b2) ^ (uint32_t)S.h(c2);
+            pm1 = (pm1 + d1) + e1;
+            pm2 = (pm2 + d2) + e2;
+            r1 = q1 & (q1 - 1u);
+            r2 = q2 & (q2 - 1u);
+        } while (r1 | r2);
         const double e1 = exp_neg(pm1, cfg.exptab);
         const double e2 = exp_neg(pm2, cfg.exptab);
         ss.T += 1;

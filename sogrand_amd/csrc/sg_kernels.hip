@@ -80,8 +80,8 @@ constexpr int kPlainThreads = 256;
 template <bool FINITE_T>
 __global__ __launch_bounds__(kPlainThreads) void plain_kernel(CubeJob job)
 {
-    __shared__ double s_delta[16 * kPlainThreads];
-    __shared__ uint16_t s_hp[16 * kPlainThreads];
+    __shared__ double s_delta[17 * kPlainThreads];
+    __shared__ uint16_t s_hp[17 * kPlainThreads];
     __shared__ uint8_t s_hd[kCube];
     __shared__ uint32_t s_sum[kPlainThreads / 64], s_max[kPlainThreads / 64];
     __shared__ ExpPair s_exp[32];
@@ -194,8 +194,8 @@ __global__ __launch_bounds__(256) void lines_kernel(const double *llr, size_t n,
                                                     double *le, uint32_t *nguess,
                                                     int32_t *status, DecodeCfg cfg)
 {
-    __shared__ double s_delta[16 * 256];
-    __shared__ uint16_t s_hp[16 * 256];
+    __shared__ double s_delta[17 * 256];
+    __shared__ uint16_t s_hp[17 * 256];
     __shared__ ExpPair s_exp[32];
     load_exp_table(s_exp, threadIdx.x);
     __syncthreads();

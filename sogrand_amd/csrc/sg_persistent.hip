@@ -136,8 +136,8 @@ template <bool FINITE_T, int S>
 __global__ __launch_bounds__(64 * kWaves) void persistent_kernel(CubeJob job, uint32_t thresh,
                                                                  uint32_t step_len)
 {
-    __shared__ double s_d[kWaves][16 * 64];
-    __shared__ uint16_t s_h[kWaves][16 * 64];
+    __shared__ double s_d[kWaves][17 * 64];
+    __shared__ uint16_t s_h[kWaves][17 * 64];
     __shared__ uint32_t s_hd[kWaves][S][128];
     __shared__ uint32_t s_cnt[kWaves][S][4]; // lines done, phase guess sum, phase max, bad
     __shared__ double s_stage_jn[kWaves][64];
@@ -427,6 +427,8 @@ __global__ __launch_bounds__(64 * kWaves) void persistent_kernel(CubeJob job, ui
                     R.h((int)rank) = (uint16_t)(hcol_of((unsigned)i) | (i << 8));
                 }
                 if (myq >= 0) {
+                    L.d(16) = 0.0; // null record read by exhausted masks in search_run
+                    L.h(16) = 0;
                     if (line_bad) {
                         atomicOr(&cnt[4 * my_slot + 3], 1u);
                         ls.nl = 0;

@@ -53,7 +53,7 @@ def test_lines_match_reference(decoder):
     assert within_chain_bound(la, g["L_APP"], kappa).all()
     assert within_chain_bound(le, g["L_E"], kappa).all()
     # well-conditioned lines meet the plain 1e-9 bar
-    good = (kappa[:, 0] <= 1e-12)
+    good = (kappa[:, 0] <= 1e-11)
     assert good.sum() > 0.4 * len(good)
     assert soft_close(la[good], g["L_APP"][good]), soft_err(la[good], g["L_APP"][good])
 
@@ -114,16 +114,21 @@ def test_cubes_2000_match_oracle(decoder, oracle, oracle_fma, backend):
     decoder.set_params(backend=backend)
     r = decoder.decode(llr, soft=True)
     bits, st, soft, cond = oracle.decode(llr, soft=True, cond=True)
-    bits_f, st_f, _ = oracle_fma.decode(llr)
+    bits_f, st_f, soft_f = oracle_fma.decode(llr, soft=True)
     stable = stable_codewords(st, st_f, bits, bits_f)
     _say("2000 cw: oracle builds agree on", int(stable.sum()))
+    assert stable.mean() >= 0.99
     assert (r.bits == bits).all()
     assert (r.stats["half_iters"] == st["half_iters"]).all()
     for f in ("NG", "NG_p", "converged"):
         assert (r.stats[f][stable] == st[f][stable]).all(), f
     well = cond <= 1e-11
     assert soft_close(r.soft[well], soft[well])
-    assert frac_within(r.soft, soft, 1e-6) >= 0.97
+    # the rest: as close to the oracle as its own FMA build is
+    for tol in (1e-9, 1e-6):
+        fg, ff = frac_within(r.soft, soft, tol), frac_within(soft_f, soft, tol)
+        _say(f"  within {tol:g}: gpu {fg:.3f} oracle-fma {ff:.3f}")
+        assert fg >= ff - 0.03
 
 
 def test_backends_identical_and_batch_independent(decoder):
