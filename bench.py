@@ -1,14 +1,22 @@
 #!/usr/bin/env python3
-"""Throughput of the gfx950 SOGRAND decoder on the (16,8)^3 cubic code.
+"""Throughput and latency of the gfx950 SOGRAND decoder on the (16,8)^3 cubic code.
 
-Workload (one "step"): decode 100,000 codewords at Eb/N0 = 2.0 dB, L=3, Imax=30,
-Tmax unlimited, FP64, LLRs resident in HBM before the timed region (generated
-on the device by the library's seeded generator). `value` is the persistent
-batched decoder's codewords/s; the plain drop-in (one workgroup per codeword)
-is timed on the same data and both decodes must give identical bits
-("BER-matched"). Prints one JSON line.
+Default workload (one "step"): decode 100,000 codewords at Eb/N0 = 2.0 dB, L=3,
+Imax=30, Tmax unlimited, FP64, LLRs resident in HBM before the timed region
+(generated on the device by the library's seeded generator). `value` is the
+persistent batched decoder's codewords/s; the plain drop-in (one workgroup per
+codeword) is timed on the same data and both decodes must give identical bits
+("BER-matched"). A short serving run at 70 % of the batched capacity adds the
+p50/p99 frame latency. Prints one JSON line.
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1]
+
+Other workloads, each printing its own JSON line:
+    --workload uneven           1,000,000 codewords, per-codeword Eb/N0 ~ U[0.5, 3.5] dB,
+                                Tmax in {unlimited, 1e5, 1e3}: cw/s of both kernels, BER/BLER
+    --workload mixed-precision  1,000,000 codewords at 2.0 and 2.5 dB: FP32 variant vs FP64
+    --workload serving          Poisson frames of U{1..256} codewords at U[1, 3] dB, offered
+                                load 10-120 % of capacity: p50/p99 latency, sustained cw/s
 """
 from __future__ import annotations
 
@@ -32,6 +40,7 @@ VALU_PEAK = 256 * 4 * 16 * 2.4e9  # lane-ops/s: CUs x SIMDs x lanes x clock
 C_GUESS = 36.0   # VALU lane-ops per guess (algorithmic budget)
 C_LINE = 3500.0  # VALU lane-ops per component decode (set-up + soft output)
 PUBLISHED_CW_S = 13.19  # the reference's published GPU throughput at 2 dB (README.md:137-143)
+CHUNK = 16384
 
 
 def log(*a):
@@ -87,51 +96,290 @@ def cpu_baseline(seed: int, ebn0: float, seconds: float) -> dict:
                       f"{dt:.1f} s on {cpu_model()}"}
 
 
-def main() -> int:
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--n-cw", type=int, default=100_000)
-    ap.add_argument("--ebn0", type=float, default=2.0)
-    ap.add_argument("--seed", type=int, default=20260)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-plain", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU: the decoder shards codewords across GPUs with no "
-              f"exchange step, and this build runs on a single MI355X (asked for {args.gpus}).",
-              file=sys.stderr)
-        return 2
+class _View:
+    """A device sub-range of a DeviceArray (pointer + shape) for the C ABI."""
 
+    def __init__(self, base, byte_off: int, shape):
+        self._p = base.data_ptr() + byte_off
+        self.shape = shape
+
+    def data_ptr(self) -> int:
+        return self._p
+
+    def is_contiguous(self) -> bool:
+        return True
+
+    def numel(self) -> int:
+        return int(np.prod(self.shape))
+
+
+def gen_device(dec, seed: int, n: int, sigma) -> object:
+    """LLRs of codewords [0, n) on the device; sigma scalar or one per codeword."""
+    from sogrand_amd import hip
+
+    sigma = np.atleast_1d(np.asarray(sigma, dtype=np.float64))
+    d = hip.DeviceArray((n, 4096))
+    for off in range(0, n, CHUNK):
+        m = min(CHUNK, n - off)
+        sig = sigma if sigma.size == 1 else np.ascontiguousarray(sigma[off:off + m])
+        dec.gen_llr_device(seed, off, m, sig, _View(d, off * 4096 * 8, (m, 4096)))
+    hip.synchronize()
+    return d
+
+
+def timed_decode(dec, d_llr, d_bits, d_stats, d_soft=None) -> tuple[float, float]:
+    """(wall seconds, kernel ms) of one decode of device-resident buffers."""
+    from sogrand_amd import hip
+
+    hip.synchronize()
+    t0 = time.perf_counter()
+    dec.decode(d_llr, out_bits=d_bits, out_stats=d_stats, out_soft=d_soft)
+    hip.synchronize()
+    wall = time.perf_counter() - t0
+    return wall, dec.last_kernel_ms()[0]
+
+
+def ber_bler(bits: np.ndarray, info: np.ndarray) -> tuple[int, int]:
+    return int(np.unpackbits(bits ^ info).sum()), int((bits != info).any(axis=1).sum())
+
+
+def wilson(k: int, n: int, z: float = 1.96) -> tuple[float, float]:
+    """95 % Wilson interval of a binomial rate."""
+    if n == 0:
+        return (0.0, 1.0)
+    p = k / n
+    d = 1 + z * z / n
+    c = (p + z * z / (2 * n)) / d
+    h = z * math.sqrt(p * (1 - p) / n + z * z / (4 * n * n)) / d
+    return (max(0.0, c - h), min(1.0, c + h))
+
+
+# ---------------------------------------------------------------------------
+# serving
+
+
+def serve_load(dec, capacity: float, load: float, seconds: float, seed: int,
+               max_inflight: int = 4) -> dict:
+    """Open-loop Poisson arrivals of frames (U{1..256} codewords, per-frame Eb/N0
+    ~ U[1, 3] dB) at `load` x capacity, through the serving loop. Latency runs
+    from a frame's scheduled arrival to its results being back in host memory."""
+    import sogrand_amd as sg
+
+    rng = np.random.default_rng(seed)
+    mean_frame = 128.5
+    lam = load * capacity / mean_frame  # frames/s
+    arrivals, t = [], 0.0
+    while t < seconds:
+        t += rng.exponential(1.0 / lam)
+        arrivals.append(t)
+    nf = len(arrivals)
+    sizes = rng.integers(1, 257, nf)
+    snr = rng.uniform(1.0, 3.0, nf)
+    offs = np.concatenate([[0], np.cumsum(sizes)])
+    n = int(offs[-1])
+    sig = np.repeat([sg.sigma_for_ebn0(float(e)) for e in snr], sizes)
+    d_llr = gen_device(dec, seed, n, sig)
+    bits = np.zeros((n, 64), np.uint8)
+    views = [_View(d_llr, int(offs[k]) * 4096 * 8, (int(sizes[k]), 4096)) for k in range(nf)]
+    done = []
+    with sg.Server(dec, hbm_budget=8 << 30, max_inflight=max_inflight) as srv:
+        t0 = srv.clock() + 0.01
+        lag = 0.0
+        for k in range(nf):
+            due = t0 + arrivals[k]
+            while True:
+                now = srv.clock()
+                if now >= due:
+                    break
+                if due - now > 0.002:
+                    time.sleep(due - now - 0.001)
+                    done += srv.poll()
+            lag = max(lag, srv.clock() - due)
+            rc = srv.submit(k, views[k], bits[offs[k]:offs[k + 1]])
+            if rc != sg.OK:
+                raise RuntimeError(f"frame {k} refused ({rc})")
+        srv.drain()
+        done += srv.poll(max_n=nf + 1)
+    assert len(done) == nf and all(c.status == sg.OK for c in done)
+    lat = np.array([c.t_done - (t0 + arrivals[c.id]) for c in done])
+    first = t0 + arrivals[0]
+    last = max(c.t_done for c in done)
+    info = sg.gen_info_bits(seed, n)
+    _, blk = ber_bler(bits, info)
+    d_llr.free()
+    return {"load": load, "offered_cw_s": round(load * capacity, 1), "frames": nf, "n_cw": n,
+            "sustained_cw_s": round(n / (last - first), 1),
+            "p50_ms": round(1e3 * float(np.percentile(lat, 50)), 3),
+            "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 3),
+            "max_client_lag_ms": round(1e3 * lag, 3), "bler": blk / n}
+
+
+def serving_capacity(dec, seed: int, n: int = 40000) -> float:
+    """Batched codewords/s on the serving frame mix (per-codeword Eb/N0 ~ U[1, 3] dB)."""
+    import sogrand_amd as sg
+    from sogrand_amd import hip
+
+    snr = np.random.default_rng(seed).uniform(1.0, 3.0, n)
+    d_llr = gen_device(dec, seed, n, [sg.sigma_for_ebn0(float(e)) for e in snr])
+    d_bits = hip.DeviceArray((n, 64), np.uint8)
+    d_st = hip.DeviceArray((n,), sg.CW_STATS_DTYPE)
+    wall, _ = timed_decode(dec, d_llr, d_bits, d_st)
+    d_llr.free()
+    return n / wall
+
+
+def workload_serving(args) -> dict:
+    import sogrand_amd as sg
+
+    dec = sg.Decoder(0, backend=sg.BACKEND_PERSISTENT)
+    cap = serving_capacity(dec, args.seed)
+    log(f"serving capacity on the frame mix: {cap:.0f} cw/s")
+    pts = []
+    for load in (0.1, 0.3, 0.5, 0.7, 0.9, 1.0, 1.2):
+        r = serve_load(dec, cap, load, args.serve_seconds, args.seed + int(load * 100))
+        log(f"load {load:.1f}: {r}")
+        pts.append(r)
+    return {"metric": "serving: p50/p99 frame latency and sustained codewords/s vs offered load",
+            "unit": "ms / codewords/s", "n_gpus": 1, "dtype": "f64",
+            "data": "synthetic frames, Poisson arrivals, U{1..256} codewords, Eb/N0 ~ U[1, 3] dB",
+            "config": {"workload": "serving loop, persistent kernel, 4 batch slots, 8 GiB budget",
+                       "seconds_per_point": args.serve_seconds},
+            "capacity_cw_s": round(cap, 1), "points": pts}
+
+
+# ---------------------------------------------------------------------------
+# uneven work and mixed precision
+
+
+def workload_uneven(args) -> dict:
+    import sogrand_amd as sg
+    from sogrand_amd import hip
+
+    n = args.n_cw if args.n_cw_set else 1_000_000
+    dec = sg.Decoder(0)
+    snr = np.random.default_rng(args.seed).uniform(0.5, 3.5, n)
+    d_llr = gen_device(dec, args.seed, n, [sg.sigma_for_ebn0(float(e)) for e in snr])
+    d_bits = hip.DeviceArray((n, 64), np.uint8)
+    d_st = hip.DeviceArray((n,), sg.CW_STATS_DTYPE)
+    info = sg.gen_info_bits(args.seed, n)
+    rows = []
+    ref_bits = None
+    for tmax, label in ((sg.UINT64_MAX, "unlimited"), (100000, "1e5"), (1000, "1e3")):
+        row = {"Tmax": label}
+        pbits = None
+        for backend, name in ((sg.BACKEND_PERSISTENT, "persistent"), (sg.BACKEND_PLAIN, "plain")):
+            dec.set_params(backend=backend, Tmax=tmax)
+            wall, _ = timed_decode(dec, d_llr, d_bits, d_st)
+            bits = d_bits.to_host()
+            row[f"{name}_cw_s"] = round(n / wall, 1)
+            if backend == sg.BACKEND_PERSISTENT:
+                st = d_st.to_host()
+                be, bl = ber_bler(bits, info)
+                row.update({"ber": be / (512.0 * n), "ber_ci95": wilson(be, 512 * n),
+                            "bler": bl / n, "bler_ci95": wilson(bl, n),
+                            "mean_half_iters": float(st["half_iters"].mean()),
+                            "half_iters_range": [int(st["half_iters"].min()),
+                                                 int(st["half_iters"].max())],
+                            "mean_NG": float(st["NG"].astype(np.float64).mean())})
+                pbits = bits
+            else:
+                row["plain_identical_bits"] = bool((bits == pbits).all())
+            log(f"Tmax {label} {name}: {n / wall:.0f} cw/s")
+        row["batched_over_plain"] = round(row["persistent_cw_s"] / row["plain_cw_s"], 3)
+        if label == "unlimited":
+            ref_bits = pbits
+        elif label == "1e5":
+            row["identical_to_unlimited"] = bool((pbits == ref_bits).all())
+        rows.append(row)
+    return {"metric": "codewords/s, uneven work (per-codeword Eb/N0 ~ U[0.5, 3.5] dB)",
+            "unit": "codewords/s", "n_gpus": 1, "dtype": "f64",
+            "data": "synthetic (seeded info bits, (16,8)^3 encoding, BPSK, AWGN)",
+            "config": {"workload": f"{n} codewords, per-codeword Eb/N0 ~ U[0.5, 3.5] dB, L=3, "
+                                   f"Imax=30", "n_cw": n},
+            "rows": rows}
+
+
+def workload_mixed(args) -> dict:
+    import sogrand_amd as sg
+    from sogrand_amd import hip
+
+    n = args.n_cw if args.n_cw_set else 1_000_000
+    n_soft = min(n, 20000)
+    dec = sg.Decoder(0, backend=sg.BACKEND_PERSISTENT)
+    rows = []
+    for snr in (2.0, 2.5):
+        seed = args.seed + int(snr * 10)
+        d_llr = gen_device(dec, seed, n, sg.sigma_for_ebn0(snr))
+        d_bits = hip.DeviceArray((n, 64), np.uint8)
+        d_st = hip.DeviceArray((n,), sg.CW_STATS_DTYPE)
+        d_soft = hip.DeviceArray((n_soft, 4096))
+        info = sg.gen_info_bits(seed, n)
+        res = {}
+        for prec, name in ((sg.FP64, "fp64"), (sg.FP32_MIXED, "fp32")):
+            dec.set_params(precision=prec)
+            wall, _ = timed_decode(dec, d_llr, d_bits, d_st)
+            bits = d_bits.to_host()
+            st = d_st.to_host()
+            timed_decode(dec, _View(d_llr, 0, (n_soft, 4096)), _View(d_bits, 0, (n_soft, 64)),
+                         _View(d_st, 0, (n_soft,)), d_soft)
+            res[name] = (n / wall, bits, st, d_soft.to_host())
+            log(f"{snr} dB {name}: {n / wall:.0f} cw/s")
+        c64, b64, s64, f64 = res["fp64"]
+        c32, b32, s32, f32 = res["fp32"]
+        be64, bl64 = ber_bler(b64, info)
+        be32, bl32 = ber_bler(b32, info)
+        hd_mis = int((np.signbit(f64) != np.signbit(f32)).sum())
+        d = np.abs(f64 - f32)
+        a = np.abs(f64)
+        buckets = {}
+        for lo, hi in ((0, 1), (1, 5), (5, 10), (10, 20), (20, np.inf)):
+            m = (a >= lo) & (a < hi)
+            if m.any():
+                buckets[f"|L|in[{lo},{hi})"] = {"share": round(float(m.mean()), 4),
+                                                "max": float(d[m].max()),
+                                                "mean": float(d[m].mean())}
+        ng64 = s64["NG"].astype(np.float64).sum()
+        rows.append({
+            "ebn0_db": snr, "fp64_cw_s": round(c64, 1), "fp32_cw_s": round(c32, 1),
+            "speedup": round(c32 / c64, 3),
+            "info_bit_mismatch_rate": float(np.unpackbits(b64 ^ b32).mean()),
+            "codewords_differing": int((b64 != b32).any(axis=1).sum()),
+            "hard_decision_mismatch_rate_all_positions": hd_mis / f64.size,
+            "ber_fp64": be64 / (512.0 * n), "ber_fp32": be32 / (512.0 * n),
+            "ber_delta": (be32 - be64) / (512.0 * n),
+            "ber_ci95_fp64": wilson(be64, 512 * n), "ber_ci95_fp32": wilson(be32, 512 * n),
+            "bler_fp64": bl64 / n, "bler_fp32": bl32 / n, "bler_delta": (bl32 - bl64) / n,
+            "bler_ci95_fp64": wilson(bl64, n), "bler_ci95_fp32": wilson(bl32, n),
+            "NG_rel_delta": float((s32["NG"].astype(np.float64).sum() - ng64) / ng64),
+            "soft_abs_err_max": float(d.max()), "soft_abs_err_mean": float(d.mean()),
+            "soft_err_by_fp64_magnitude": buckets, "soft_sample_cw": n_soft})
+        for x in (d_llr, d_bits, d_st, d_soft):
+            x.free()
+    return {"metric": "FP32 variant vs FP64: mismatch, BER/BLER delta, soft-output error",
+            "unit": "codewords/s", "n_gpus": 1, "dtype": "f32 metrics / f64 found mass",
+            "data": "synthetic (seeded info bits, (16,8)^3 encoding, BPSK, AWGN)",
+            "config": {"workload": f"{n} codewords at 2.0 and 2.5 dB, persistent kernel",
+                       "n_cw": n}, "rows": rows}
+
+
+# ---------------------------------------------------------------------------
+# headline
+
+
+def workload_headline(args) -> dict:
     import sogrand_amd as sg
     from sogrand_amd import hip
 
     n = args.n_cw
     dec = sg.Decoder(0, backend=sg.BACKEND_PERSISTENT)
-    sigma = np.array([sg.sigma_for_ebn0(args.ebn0)])
-    d_llr = hip.DeviceArray((n, 4096))
+    d_llr = gen_device(dec, args.seed, n, sg.sigma_for_ebn0(args.ebn0))
     d_bits = hip.DeviceArray((n, 64), np.uint8)
     d_stats = hip.DeviceArray((n,), sg.CW_STATS_DTYPE)
-    chunk = 16384
-    for off in range(0, n, chunk):
-        m = min(chunk, n - off)
-        # device generator writes straight into HBM (codeword c depends only on (seed, c))
-        sub = _View(d_llr, off * 4096 * 8, (m, 4096))
-        dec.gen_llr_device(args.seed, off, m, sigma, sub)
-    hip.synchronize()
     log(f"generated {n} codewords at {args.ebn0} dB on the device")
 
     def run(backend: int) -> tuple[float, float]:
         dec.set_params(backend=backend)
-        hip.synchronize()
-        t0 = time.perf_counter()
-        dec.decode(d_llr, out_bits=d_bits, out_stats=d_stats)
-        hip.synchronize()
-        wall = time.perf_counter() - t0
-        kms, _ = dec.last_kernel_ms()
-        return wall, kms
+        return timed_decode(dec, d_llr, d_bits, d_stats)
 
     for w in range(args.warmup):
         run(sg.BACKEND_PERSISTENT)
@@ -151,16 +399,16 @@ def main() -> int:
     plain = None
     same_bits = None
     if not args.no_plain:
-        wall_p, kms_p = run(sg.BACKEND_PLAIN)
+        wall_p, _ = run(sg.BACKEND_PLAIN)
         plain = n / wall_p
         same_bits = bool((d_bits.to_host() == bits_b).all())
         same_stats = bool((d_stats.to_host() == st).all())
         log(f"plain: {wall_p * 1e3:.1f} ms wall -> {plain:.0f} cw/s; identical bits {same_bits}, "
             f"identical stats {same_stats}")
+    d_llr.free()
 
     info = sg.gen_info_bits(args.seed, n)
-    bit_err = int(np.unpackbits(bits_b ^ info).sum())
-    blk_err = int((bits_b != info).any(axis=1).sum())
+    bit_err, blk_err = ber_bler(bits_b, info)
     ng = float(st["NG"].astype(np.float64).sum())
     lines = 256.0 * float(st["half_iters"].astype(np.float64).sum())
     kernel_s = (sum(kmss) / len(kmss)) / 1e3
@@ -202,27 +450,48 @@ def main() -> int:
             "kernel_ms": round(kernel_s * 1e3, 2),
         },
     }
+    if not args.no_serve:
+        cap = serving_capacity(dec, args.seed + 1)
+        r = serve_load(dec, cap, 0.7, args.serve_seconds, args.seed + 2)
+        log(f"serving at 70 % of {cap:.0f} cw/s: {r}")
+        out["latency"] = {"p50_ms": r["p50_ms"], "p99_ms": r["p99_ms"],
+                          "workload": "serving loop, Poisson frames of U{1..256} codewords at "
+                                      "Eb/N0 ~ U[1, 3] dB, offered 70 % of batched capacity",
+                          "capacity_cw_s": round(cap, 1), "sustained_cw_s": r["sustained_cw_s"],
+                          "frames": r["frames"]}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.seed, args.ebn0, args.cpu_seconds)
-    print(json.dumps(out), flush=True)
+    return out
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="headline",
+                    choices=["headline", "uneven", "mixed-precision", "serving"])
+    ap.add_argument("--n-cw", type=int, default=None)
+    ap.add_argument("--ebn0", type=float, default=2.0)
+    ap.add_argument("--seed", type=int, default=20260)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--serve-seconds", type=float, default=3.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: the decoder shards codewords across GPUs with no "
+              f"exchange step, and this build runs on a single MI355X (asked for {args.gpus}).",
+              file=sys.stderr)
+        return 2
+    args.n_cw_set = args.n_cw is not None
+    if args.n_cw is None:
+        args.n_cw = 100_000
+    fn = {"headline": workload_headline, "uneven": workload_uneven,
+          "mixed-precision": workload_mixed, "serving": workload_serving}[args.workload]
+    print(json.dumps(fn(args)), flush=True)
     return 0
-
-
-class _View:
-    """A device sub-range of a DeviceArray (pointer + shape) for the C ABI."""
-
-    def __init__(self, base, byte_off: int, shape):
-        self._p = base.data_ptr() + byte_off
-        self.shape = shape
-
-    def data_ptr(self) -> int:
-        return self._p
-
-    def is_contiguous(self) -> bool:
-        return True
-
-    def numel(self) -> int:
-        return int(np.prod(self.shape))
 
 
 if __name__ == "__main__":

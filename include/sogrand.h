@@ -144,6 +144,55 @@ int sogrand_gen_llr(uint64_t seed, uint64_t cw_offset, size_t n_cw, const double
 int sogrand_gen_llr_device(sogrand_ctx *ctx, uint64_t seed, uint64_t cw_offset, size_t n_cw,
                            const double *sigma, size_t n_sigma, double *llr, void *hip_stream);
 
+/* ---- serving loop: continuous batching of decode requests ----
+ * No counterpart in the reference, whose nearest relative is the multi-stream
+ * CUDA decoder (SOGRAND_CUDA/cubic_decoder.cu:1145-1179, batches written out of
+ * order). A worker thread keeps up to max_inflight batches decoding at once,
+ * each on its own stream with a fixed share of hbm_budget (no allocation after
+ * create). Whenever a batch slot frees up, every waiting request that fits is
+ * gathered into the next batch, so requests that arrive while others decode are
+ * admitted as slots free up and their work overlaps on the device. Results are
+ * reported by request id; each codeword decodes exactly as it does alone. The
+ * server copies the context's parameters at create; the context must outlive it. */
+typedef struct sogrand_server sogrand_server;
+
+typedef struct sogrand_server_cfg {
+    size_t hbm_budget;       /* device bytes for all batch slots (default 4 GiB)          */
+    uint32_t max_inflight;   /* batch slots = concurrent launches, 1..16 (default 4)       */
+    uint32_t max_batch_cw;   /* codewords per batch (0 = what the budget allows, <= 65536) */
+    uint32_t max_pending_cw; /* waiting codewords before submit refuses (default 2^20)     */
+    int32_t with_soft;       /* reserve room for soft outputs (default 0)                  */
+} sogrand_server_cfg;
+
+typedef struct sogrand_completion {
+    uint64_t id;     /* the id given to sogrand_server_submit                         */
+    int32_t status;  /* SOGRAND_OK, SOGRAND_ENONFINITE or SOGRAND_EHIP                 */
+    uint32_t n_cw;
+    double t_submit; /* seconds on the server clock: submitted, batch launched, results */
+    double t_start;  /* copied back                                                     */
+    double t_done;
+} sogrand_completion;
+
+void sogrand_server_cfg_default(sogrand_server_cfg *cfg);
+int sogrand_server_create(sogrand_server **out, sogrand_ctx *ctx, const sogrand_server_cfg *cfg);
+/* Codewords per batch slot (the largest request the server accepts). */
+int sogrand_server_batch_cw(const sogrand_server *srv, size_t *batch_cw);
+/* Queue a request of n_cw codewords (llr: host or device memory, 4096 f64 each).
+ * Outputs (host or device) are written before its completion is reported; the
+ * input must stay valid until then. soft_out needs with_soft. Returns
+ * SOGRAND_EBUSY when max_pending_cw codewords are already waiting. */
+int sogrand_server_submit(sogrand_server *srv, uint64_t id, const double *llr, size_t n_cw,
+                          uint8_t *info_bits, double *soft_out, sogrand_cw_stats *stats);
+/* Move up to max completions into out (oldest first); waits up to wait_ms for
+ * the first one when none is ready. */
+int sogrand_server_poll(sogrand_server *srv, sogrand_completion *out, size_t max, size_t *n_out,
+                        int wait_ms);
+/* Block until every submitted request has completed. */
+int sogrand_server_drain(sogrand_server *srv);
+double sogrand_server_clock(const sogrand_server *srv);
+/* Finishes the queued work, then frees the server. */
+void sogrand_server_destroy(sogrand_server *srv);
+
 #ifdef __cplusplus
 }
 #endif

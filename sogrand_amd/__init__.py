@@ -45,7 +45,10 @@ EXPORTED = [
     "sogrand_params_default", "sogrand_strerror", "sogrand_ctx_create", "sogrand_ctx_set_params",
     "sogrand_ctx_destroy", "sogrand_decode_cubic", "sogrand_bitso_lines", "sogrand_last_kernel_ms",
     "sogrand_sigma_for_ebn0", "sogrand_gen_info_bits", "sogrand_encode_cubes", "sogrand_gen_llr",
-    "sogrand_gen_llr_device", "sogrand_schedule_table",
+    "sogrand_gen_llr_device", "sogrand_schedule_table", "sogrand_server_cfg_default",
+    "sogrand_server_create", "sogrand_server_batch_cw", "sogrand_server_submit",
+    "sogrand_server_poll", "sogrand_server_drain", "sogrand_server_clock",
+    "sogrand_server_destroy",
 ]
 
 
@@ -69,6 +72,27 @@ class Params(C.Structure):
         ("waves_per_cu", C.c_int32),
         ("lane_batch", C.c_int32),
         ("step_len", C.c_int32),
+    ]
+
+
+class ServerCfg(C.Structure):
+    _fields_ = [
+        ("hbm_budget", C.c_size_t),
+        ("max_inflight", C.c_uint32),
+        ("max_batch_cw", C.c_uint32),
+        ("max_pending_cw", C.c_uint32),
+        ("with_soft", C.c_int32),
+    ]
+
+
+class Completion(C.Structure):
+    _fields_ = [
+        ("id", C.c_uint64),
+        ("status", C.c_int32),
+        ("n_cw", C.c_uint32),
+        ("t_submit", C.c_double),
+        ("t_start", C.c_double),
+        ("t_done", C.c_double),
     ]
 
 
@@ -105,6 +129,18 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.sogrand_gen_llr_device.argtypes = [vp, C.c_uint64, C.c_uint64, C.c_size_t, vp, C.c_size_t,
                                            vp, vp]
     lib.sogrand_schedule_table.argtypes = [C.c_int32, vp]
+    lib.sogrand_server_cfg_default.argtypes = [C.POINTER(ServerCfg)]
+    lib.sogrand_server_cfg_default.restype = None
+    lib.sogrand_server_create.argtypes = [C.POINTER(vp), vp, C.POINTER(ServerCfg)]
+    lib.sogrand_server_batch_cw.argtypes = [vp, C.POINTER(C.c_size_t)]
+    lib.sogrand_server_submit.argtypes = [vp, C.c_uint64, vp, C.c_size_t, vp, vp, vp]
+    lib.sogrand_server_poll.argtypes = [vp, C.POINTER(Completion), C.c_size_t,
+                                        C.POINTER(C.c_size_t), C.c_int]
+    lib.sogrand_server_drain.argtypes = [vp]
+    lib.sogrand_server_clock.argtypes = [vp]
+    lib.sogrand_server_clock.restype = C.c_double
+    lib.sogrand_server_destroy.argtypes = [vp]
+    lib.sogrand_server_destroy.restype = None
     _lib = lib
     return lib
 
@@ -230,6 +266,83 @@ class Decoder:
         ns = sig.shape[0] if isinstance(sig, np.ndarray) else sig.numel()
         _check(self._lib.sogrand_gen_llr_device(self._h, seed, cw_offset, n, _ptr(sig), ns,
                                                 _ptr(out), stream), "gen_llr_device")
+
+
+class Server:
+    """Continuous-batching serving loop on a Decoder (sogrand_server).
+
+    submit() queues a frame of codewords (host or device LLRs) and returns at
+    once; poll() returns completion records (id, status, n_cw and the server
+    clock at submit, launch and completion). Output arrays passed to submit()
+    are filled before the frame's completion is reported; the caller keeps the
+    input and outputs alive until then."""
+
+    def __init__(self, decoder: Decoder, hbm_budget: int | None = None, max_inflight: int = 4,
+                 max_batch_cw: int = 0, max_pending_cw: int = 1 << 20, with_soft: bool = False):
+        lib = decoder._lib
+        self._lib = lib
+        cfg = ServerCfg()
+        lib.sogrand_server_cfg_default(C.byref(cfg))
+        if hbm_budget is not None:
+            cfg.hbm_budget = hbm_budget
+        cfg.max_inflight = max_inflight
+        cfg.max_batch_cw = max_batch_cw
+        cfg.max_pending_cw = max_pending_cw
+        cfg.with_soft = 1 if with_soft else 0
+        h = C.c_void_p()
+        _check(lib.sogrand_server_create(C.byref(h), decoder._h, C.byref(cfg)), "server_create")
+        self._h = h
+        self._keep = {}
+        b = C.c_size_t()
+        _check(lib.sogrand_server_batch_cw(h, C.byref(b)), "server_batch_cw")
+        self.batch_cw = b.value
+
+    def submit(self, rid: int, llr, info_out, stats_out=None, soft_out=None) -> int:
+        """Queue frame rid; returns 0, or EBUSY when the admission queue is full."""
+        n = llr.shape[0] if isinstance(llr, np.ndarray) else llr.numel() // 4096
+        rc = self._lib.sogrand_server_submit(self._h, rid, _ptr(llr), n, _ptr(info_out),
+                                             _ptr(soft_out), _ptr(stats_out))
+        if rc == EBUSY:
+            return rc
+        _check(rc, "server_submit")
+        self._keep[rid] = (llr, info_out, stats_out, soft_out)
+        return OK
+
+    def poll(self, max_n: int = 1024, wait_ms: int = 0) -> list:
+        buf = (Completion * max_n)()
+        k = C.c_size_t()
+        _check(self._lib.sogrand_server_poll(self._h, buf, max_n, C.byref(k), wait_ms),
+               "server_poll")
+        out = []
+        for i in range(k.value):
+            c = buf[i]
+            self._keep.pop(c.id, None)
+            out.append(c)
+        return out
+
+    def drain(self) -> None:
+        _check(self._lib.sogrand_server_drain(self._h), "server_drain")
+
+    def clock(self) -> float:
+        return self._lib.sogrand_server_clock(self._h)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.sogrand_server_destroy(self._h)
+            self._h = None
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
 
 
 # ---- host-side data helpers (no GPU needed) ----

@@ -11,6 +11,7 @@ from __future__ import annotations
 import os
 import shutil
 import subprocess
+from concurrent.futures import ThreadPoolExecutor
 import sys
 
 PKG = os.path.dirname(os.path.abspath(__file__))
@@ -34,7 +35,9 @@ COMMON = [
     "-Wall",
 ]
 
-LIB_SOURCES = ["sg_kernels.hip", "sg_persistent.hip", "sg_api.cpp", "sg_schedule.cpp"]
+LIB_SOURCES = [
+    "sg_kernels.hip", "sg_persistent.hip", "sg_api.cpp", "sg_schedule.cpp", "sg_serve.cpp",
+]
 
 
 def _run(cmd: list[str]) -> None:
@@ -48,18 +51,19 @@ def _run(cmd: list[str]) -> None:
 
 
 def build(verbose: bool = False) -> str:
-    objs = []
     tmpdir = os.path.join(PKG, "_obj")
     os.makedirs(tmpdir, exist_ok=True)
+    cmds, objs = [], []
     for src in LIB_SOURCES:
         obj = os.path.join(tmpdir, os.path.splitext(src)[0] + ".o")
-        cmd = [HIPCC, *COMMON, "-c", os.path.join(CSRC, src), "-o", obj]
-        if verbose:
-            print(" ".join(cmd))
-        _run(cmd)
+        cmds.append([HIPCC, *COMMON, "-pthread", "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
+        if verbose:
+            print(" ".join(cmds[-1]))
+    with ThreadPoolExecutor(max_workers=min(len(cmds), 6)) as ex:
+        list(ex.map(_run, cmds))
     tmp = LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp])
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", *objs, "-o", tmp])
     os.replace(tmp, LIB)
     cli_cmd = [HIPCC, *COMMON, os.path.join(CSRC, "cubic_decoder_main.cpp"), "-o", CLI + ".tmp",
                "-L" + PKG, "-lsogrand", "-Wl,-rpath,$ORIGIN"]

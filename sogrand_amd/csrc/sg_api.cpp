@@ -12,37 +12,13 @@
 #include <vector>
 
 #include "sg_common.h"
+#include "sg_ctx.h"
 #include "sg_launch.h"
 #include "sg_schedule.h"
 
 #pragma clang fp contract(off)
 
-namespace {
-
-template <class T> struct DevBuf {
-    T *p = nullptr;
-    size_t n = 0;
-    hipError_t reserve(size_t want)
-    {
-        if (want <= n) return hipSuccess;
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-        hipError_t e = hipMalloc(&p, sizeof(T) * want);
-        if (e != hipSuccess) {
-            p = nullptr;
-            return e;
-        }
-        n = want;
-        return hipSuccess;
-    }
-    void release()
-    {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-};
+namespace sg {
 
 bool is_device_ptr(const void *ptr)
 {
@@ -56,24 +32,71 @@ bool is_device_ptr(const void *ptr)
     return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
 }
 
-} // namespace
+} // namespace sg
 
-struct sogrand_ctx {
-    int device = 0;
-    sogrand_params p{};
-    hipStream_t own_stream = nullptr;
-    uint16_t *d_sched = nullptr;
-    DevBuf<double> le_scratch, llr_stage, soft_stage, sigma_stage;
-    DevBuf<uint8_t> info_stage;
-    DevBuf<uint32_t> stats_stage, queue;
-    DevBuf<int32_t> status;
-    DevBuf<uint32_t> nguess_stage;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    double last_ms = 0.0;
-    uint32_t last_launches = 0;
-    int cu_count = 256;
-    std::mutex mu;
-};
+using sg::effective_tmax;
+using sg::is_device_ptr;
+
+namespace sg {
+
+LaunchPlan plan_launch(const sogrand_ctx *ctx, const sogrand_params &p, size_t n)
+{
+    LaunchPlan lp{};
+    lp.persistent = p.backend == SOGRAND_BACKEND_PERSISTENT;
+    if (lp.persistent) {
+        const int slots = p.wave_slots > 0 ? p.wave_slots : 1;
+        const int wpb = kPersistentWaves;
+        // a persistent grid must be resident all at once: never more workgroups
+        // per CU than the occupancy the kernel's registers and LDS allow
+        const uint64_t tmax = effective_tmax(p.Tmax);
+        const int occ = persistent_blocks_per_cu(slots, tmax < (1ull << 17),
+                                                 p.precision == SOGRAND_FP32_MIXED);
+        int bpc = p.waves_per_cu > 0 ? (p.waves_per_cu + wpb - 1) / wpb : occ;
+        if (bpc > occ) bpc = occ;
+        int blocks = ctx->cu_count * bpc;
+        // never more workers than cubes
+        const size_t max_blocks = (n + (size_t)wpb * slots - 1) / ((size_t)wpb * slots) + 1;
+        if ((size_t)blocks > max_blocks) blocks = (int)max_blocks;
+        lp.blocks = blocks;
+        lp.slots = slots;
+        lp.le_slots = (size_t)blocks * wpb * slots;
+    } else {
+        lp.blocks = (int)plain_scratch_slots(n);
+        lp.slots = 1;
+        lp.le_slots = (size_t)lp.blocks;
+    }
+    return lp;
+}
+
+size_t max_le_slots(size_t n) { return n + 2 * (size_t)kPersistentWaves * 2; }
+
+void fill_job(CubeJob &job, const sogrand_ctx *ctx, const sogrand_params &p)
+{
+    job.sched = ctx->d_sched;
+    job.tmax = effective_tmax(p.Tmax);
+    job.thres = 1.0 - p.p_ET;
+    job.alpha = p.alpha;
+    job.L = p.L;
+    job.Imax = p.Imax;
+    job.f32 = p.precision == SOGRAND_FP32_MIXED;
+}
+
+hipError_t launch_cubes(const sogrand_params &p, const LaunchPlan &lp, const CubeJob &job,
+                        hipStream_t st, hipEvent_t ev_start)
+{
+    if (lp.persistent) {
+        hipError_t e = launch_fill_u32(job.queue, 0u, 1, st);
+        if (e != hipSuccess) return e;
+        if (ev_start) (void)hipEventRecord(ev_start, st);
+        const uint32_t batch = p.lane_batch > 0 ? (uint32_t)p.lane_batch : 4u;
+        const uint32_t step = p.step_len > 0 ? (uint32_t)p.step_len : 64u;
+        return launch_persistent(job, kPersistentWaves, lp.blocks, lp.slots, batch, step, st);
+    }
+    if (ev_start) (void)hipEventRecord(ev_start, st);
+    return launch_plain(job, lp.blocks, st);
+}
+
+} // namespace sg
 
 extern "C" {
 
@@ -211,7 +234,6 @@ void sogrand_ctx_destroy(sogrand_ctx *ctx)
     delete ctx;
 }
 
-static uint64_t effective_tmax(uint64_t t) { return t == 0 ? 0x7fffffffull : t; }
 
 // Decode one chunk whose buffers are all on the device.
 static int decode_device_chunk(sogrand_ctx *ctx, const double *d_llr, size_t n, uint8_t *d_info,
@@ -226,43 +248,13 @@ static int decode_device_chunk(sogrand_ctx *ctx, const double *d_llr, size_t n, 
     job.stats = d_stats;
     job.status = ctx->status.p;
     job.n_cw = n;
-    job.sched = ctx->d_sched;
-    job.tmax = effective_tmax(p.Tmax);
-    job.thres = 1.0 - p.p_ET;
-    job.alpha = p.alpha;
-    job.L = p.L;
-    job.Imax = p.Imax;
-    hipError_t e;
-    if (p.backend == SOGRAND_BACKEND_PERSISTENT) {
-        const int slots = p.wave_slots > 0 ? p.wave_slots : 1;
-        const int wpb = sg::kPersistentWaves;
-        // a persistent grid must be resident all at once: never more workgroups
-        // per CU than the occupancy the kernel's registers and LDS allow
-        const int occ = sg::persistent_blocks_per_cu(slots, job.tmax < (1ull << 17));
-        int bpc = p.waves_per_cu > 0 ? (p.waves_per_cu + wpb - 1) / wpb : occ;
-        if (bpc > occ) bpc = occ;
-        int blocks = ctx->cu_count * bpc;
-        // never more workers than cubes
-        const size_t max_blocks = (n * 1 + (size_t)wpb * slots - 1) / ((size_t)wpb * slots) + 1;
-        if ((size_t)blocks > max_blocks) blocks = (int)max_blocks;
-        const size_t total_slots = (size_t)blocks * wpb * slots;
-        if (ctx->le_scratch.reserve(total_slots * sg::kCube) != hipSuccess) return SOGRAND_ENOMEM;
-        if (ctx->queue.reserve(4) != hipSuccess) return SOGRAND_ENOMEM;
-        job.le_scratch = ctx->le_scratch.p;
-        job.queue = ctx->queue.p;
-        if (sg::launch_fill_u32(ctx->queue.p, 0u, 4, st) != hipSuccess) return SOGRAND_EHIP;
-        (void)hipEventRecord(ctx->ev0, st);
-        const uint32_t batch = p.lane_batch > 0 ? (uint32_t)p.lane_batch : 4u;
-        const uint32_t step = p.step_len > 0 ? (uint32_t)p.step_len : 64u;
-        e = sg::launch_persistent(job, wpb, blocks, slots, batch, step, st);
-    } else {
-        const size_t slots = sg::plain_scratch_slots(n);
-        if (ctx->le_scratch.reserve(slots * sg::kCube) != hipSuccess) return SOGRAND_ENOMEM;
-        job.le_scratch = ctx->le_scratch.p;
-        (void)hipEventRecord(ctx->ev0, st);
-        e = sg::launch_plain(job, (int)slots, st);
-    }
-    if (e != hipSuccess) return SOGRAND_EHIP;
+    sg::fill_job(job, ctx, p);
+    const sg::LaunchPlan lp = sg::plan_launch(ctx, p, n);
+    if (ctx->le_scratch.reserve(lp.le_slots * sg::kCube) != hipSuccess) return SOGRAND_ENOMEM;
+    if (ctx->queue.reserve(4) != hipSuccess) return SOGRAND_ENOMEM;
+    job.le_scratch = ctx->le_scratch.p;
+    job.queue = ctx->queue.p;
+    if (sg::launch_cubes(p, lp, job, st, ctx->ev0) != hipSuccess) return SOGRAND_EHIP;
     (void)hipEventRecord(ctx->ev1, st);
     if (hipEventSynchronize(ctx->ev1) != hipSuccess) return SOGRAND_EHIP;
     float t = 0.f;
@@ -370,7 +362,7 @@ int sogrand_bitso_lines(sogrand_ctx *ctx, const double *llr, size_t n_lines, dou
         return SOGRAND_EHIP;
     if (sg::launch_lines(d_in, n_lines, d_la, d_le, ctx->nguess_stage.p, ctx->status.p,
                          ctx->d_sched, effective_tmax(ctx->p.Tmax), 1.0 - ctx->p.p_ET, ctx->p.L,
-                         st) != hipSuccess)
+                         ctx->p.precision == SOGRAND_FP32_MIXED, st) != hipSuccess)
         return SOGRAND_EHIP;
     if (hipStreamSynchronize(st) != hipSuccess) return SOGRAND_EHIP;
     if (L_APP && hipMemcpy(L_APP, d_la, n_lines * 16 * sizeof(double), hipMemcpyDefault) !=

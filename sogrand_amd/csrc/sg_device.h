@@ -64,16 +64,39 @@ __device__ __forceinline__ double exp_neg(double pm, const ExpPair *tab)
     return under ? 0.0 : v;
 }
 
-// Per-lane LDS record arrays: delta (f64) and (H column | natural position << 8),
+// Arithmetic of the per-guess path metrics.
+//   PrecF64: everything in double, bit-exact with the reference.
+//   PrecF32: reliabilities sorted as floats, path-metric increments stored and
+//            summed as floats, exp(-pm) of each guess in single precision; the
+//            1 - found-mass chain (P_notGuess), PM of the hard decision, list
+//            weights (exp(-pm) of hits, recomputed in double), confidence and
+//            soft output stay in double.
+struct PrecF64 {
+    using Rec = double;
+    static constexpr bool kF32 = false;
+};
+struct PrecF32 {
+    using Rec = float;
+    static constexpr bool kF32 = true;
+};
+
+// exp(-pm) of one guess in single precision (v_exp_f32 on -pm log2 e), widened.
+__device__ __forceinline__ double exp_neg_f32(float pm)
+{
+    return (double)__builtin_amdgcn_exp2f(pm * -1.44269504088896341f);
+}
+
+// Per-lane LDS record arrays: delta (Rec) and (H column | natural position << 8),
 // 17 records per slot: sorted positions 0..15 and a null record 16.
-struct LineLds {
-    double *delta;
+template <class Rec> struct LineLdsT {
+    Rec *delta;
     uint16_t *hp;
     int nslot;
     int slot;
-    __device__ double &d(int b) const { return delta[b * nslot + slot]; }
+    __device__ Rec &d(int b) const { return delta[b * nslot + slot]; }
     __device__ uint16_t &h(int b) const { return hp[b * nslot + slot]; }
 };
+using LineLds = LineLdsT<double>;
 
 // JacLog (:762).
 __device__ __forceinline__ double jaclog(double x)
@@ -172,6 +195,35 @@ __device__ __forceinline__ double setup_with_ties(const LineLds S)
     return pm_hd;
 }
 
+// Move the natural-order records d(i) to their sorted positions rank_i (4 bits
+// per position in `rank`), one permutation cycle at a time, and write the H
+// column | natural position records alongside.
+template <class Rec>
+__device__ __forceinline__ void place_by_rank(const LineLdsT<Rec> &S, uint64_t rank)
+{
+    uint32_t placed = 0;
+#pragma unroll 1
+    for (int start = 0; start < 16; ++start) {
+        if ((placed >> start) & 1u) continue;
+        Rec carry = S.d(start);
+        int from = start;
+#pragma unroll 1
+        for (;;) {
+            const int to = (int)((rank >> (4 * from)) & 15u);
+            placed |= 1u << to;
+            S.h(to) = (uint16_t)(hcol_of((unsigned)from) | (from << 8));
+            if (to == start) {
+                S.d(to) = carry;
+                break;
+            }
+            const Rec t = S.d(to);
+            S.d(to) = carry;
+            carry = t;
+            from = to;
+        }
+    }
+}
+
 // Line set-up (sogrand_main_logic :598-622): hard decision, PM of the hard
 // decision in natural order, sort, intercept, and the sorted LDS records.
 //
@@ -240,34 +292,77 @@ __device__ __forceinline__ bool line_setup(const Src &src, const LineLds &S, uin
             pm_hd += jn;
             S.d(i) = jaclog(ai) - jn;
         }
-        // move delta_i to sorted position rank_i, one permutation cycle at a time
-        uint32_t placed = 0;
-#pragma unroll 1
-        for (int start = 0; start < 16; ++start) {
-            if ((placed >> start) & 1u) continue;
-            double carry = S.d(start);
-            int from = start;
-#pragma unroll 1
-            for (;;) {
-                const int to = (int)((rank >> (4 * from)) & 15u);
-                placed |= 1u << to;
-                S.h(to) = (uint16_t)(hcol_of((unsigned)from) | (from << 8));
-                if (to == start) {
-                    S.d(to) = carry;
-                    break;
-                }
-                const double t = S.d(to);
-                S.d(to) = carry;
-                carry = t;
-                from = to;
-            }
-        }
+        place_by_rank(S, rank);
     } else {
         pm_hd = setup_with_ties(S);
     }
     S.d(16) = 0.0; // null record read by exhausted masks in search_run
     S.h(16) = 0;
     return true;
+}
+
+// Set-up of the single-precision variant: reliabilities ranked as floats (equal
+// floats ordered by position), IC from the two float order statistics, PM_HD
+// in double in natural order, increments computed in double and rounded to
+// float. Returns false if an input is not finite.
+template <class Src>
+__device__ __forceinline__ bool line_setup_f32(const Src &src, const LineLdsT<float> &S,
+                                               uint32_t &hd, uint32_t &s0, double &pm_hd,
+                                               int32_t &ic)
+{
+    bool finite = true;
+    hd = 0;
+#pragma unroll 1
+    for (int i = 0; i < 16; ++i) {
+        const double x = src(i);
+        finite = finite && __builtin_isfinite(x);
+        if (!(x > 0.0)) hd |= 1u << i; // HardDec (:758)
+        S.d(i) = (float)fabs(x);
+    }
+    s0 = syndrome16(hd);
+    if (!finite) return false;
+    uint64_t rank = 0;
+    float a_first = 0.0f, a_eighth = 0.0f;
+#pragma unroll 1
+    for (int i = 0; i < 16; ++i) {
+        const float ai = S.d(i);
+        uint32_t r = 0;
+#pragma unroll 4
+        for (int j = 0; j < 16; ++j) {
+            const float aj = S.d(j);
+            r += (aj < ai || (aj == ai && j < i)) ? 1u : 0u;
+        }
+        rank |= (uint64_t)r << (4 * i);
+        if (r == 0) a_first = ai;
+        if (r == 7) a_eighth = ai;
+    }
+    const double beta = ((double)a_eighth - (double)a_first) / 7.0;
+    const int32_t rr = cvtt_i32(round((double)a_first / beta - 1.0));
+    ic = rr > 0 ? rr : 0;
+    pm_hd = 0.0;
+#pragma unroll 1
+    for (int i = 0; i < 16; ++i) {
+        const double ai = fabs(src(i));
+        const double jn = jaclog(-ai);
+        pm_hd += jn;
+        S.d(i) = (float)(jaclog(ai) - jn);
+    }
+    place_by_rank(S, rank);
+    S.d(16) = 0.0f;
+    S.h(16) = 0;
+    return true;
+}
+
+// Set-up in the arithmetic of P.
+template <class P, class Src>
+__device__ __forceinline__ bool line_setup_p(const Src &src, const LineLdsT<typename P::Rec> &S,
+                                             uint32_t &hd, uint32_t &s0, double &pm_hd,
+                                             int32_t &ic)
+{
+    if constexpr (P::kF32)
+        return line_setup_f32(src, S, hd, s0, pm_hd, ic);
+    else
+        return line_setup(src, S, hd, s0, pm_hd, ic);
 }
 
 // Record a list entry and apply the stopping rule (:628-641, :674-686, getLConf :767-774).
@@ -328,11 +423,14 @@ __device__ __forceinline__ void search_init(SearchState &ss, LineList &ls, uint3
 // chains; the guesses are still accounted strictly in schedule order (the
 // second is dropped if the first ends the search). Masks stream 8 per 16-byte
 // load with the next load issued one chunk ahead.
-template <bool FINITE_T>
-__device__ __forceinline__ void search_run(SearchState &ss, LineList &ls, const LineLds &S,
-                                           uint32_t hd, uint32_t s0, double pm_hd, int32_t ic,
+template <bool FINITE_T, class P = PrecF64>
+__device__ __forceinline__ void search_run(SearchState &ss, LineList &ls,
+                                           const LineLdsT<typename P::Rec> &S, uint32_t hd,
+                                           uint32_t s0, double pm_hd, int32_t ic,
                                            const DecodeCfg &cfg, uint32_t max_steps)
 {
+    using Rec = typename P::Rec;
+    const Rec pm_base = (Rec)pm_hd;
 #pragma unroll 1
     for (uint32_t step = 0; step < max_steps && !ss.done; step += 2) {
         const uint32_t t = ss.t; // always even here
@@ -371,14 +469,14 @@ __device__ __forceinline__ void search_run(SearchState &ss, LineList &ls, const 
         // syndromes, two set bits of each pattern per iteration; an exhausted
         // mask reads the null record 16 (delta +0.0, H column 0), which leaves
         // both sums unchanged, so no lane needs a select or a branch
-        double pm1 = pm_hd, pm2 = pm_hd;
+        Rec pm1 = pm_base, pm2 = pm_base;
         uint32_t syn1 = s0, syn2 = s0;
         uint32_t r1 = m1, r2 = take2 ? m2 : 0u;
         do {
             const uint32_t q1 = r1 & (r1 - 1u), q2 = r2 & (r2 - 1u);
             const int b1 = __builtin_ctz(r1 | 0x10000u), c1 = __builtin_ctz(q1 | 0x10000u);
             const int b2 = __builtin_ctz(r2 | 0x10000u), c2 = __builtin_ctz(q2 | 0x10000u);
-            const double d1 = S.d(b1), e1 = S.d(c1), d2 = S.d(b2), e2 = S.d(c2);
+            const Rec d1 = S.d(b1), e1 = S.d(c1), d2 = S.d(b2), e2 = S.d(c2);
             syn1 ^= (uint32_t)S.h(b1) ^ (uint32_t)S.h(c1);
             syn2 ^= (uint32_t)S.h(b2) ^ (uint32_t)S.h(c2);
             pm1 = (pm1 + d1) + e1;
@@ -386,12 +484,21 @@ __device__ __forceinline__ void search_run(SearchState &ss, LineList &ls, const 
             r1 = q1 & (q1 - 1u);
             r2 = q2 & (q2 - 1u);
         } while (r1 | r2);
-        const double e1 = exp_neg(pm1, cfg.exptab);
-        const double e2 = exp_neg(pm2, cfg.exptab);
+        const bool hit1 = (syn1 & 0xffu) == 0, hit2 = (syn2 & 0xffu) == 0;
+        double e1, e2;
+        if (P::kF32) {
+            // single-precision weights for the found-mass chain; a hit's list
+            // weight is taken in double from its (single-precision) metric
+            e1 = hit1 ? exp_neg((double)pm1, cfg.exptab) : exp_neg_f32((float)pm1);
+            e2 = hit2 ? exp_neg((double)pm2, cfg.exptab) : exp_neg_f32((float)pm2);
+        } else {
+            e1 = exp_neg((double)pm1, cfg.exptab);
+            e2 = exp_neg((double)pm2, cfg.exptab);
+        }
         ss.T += 1;
         ss.t = t + 1;
         ss.pnot -= e1;
-        if ((syn1 & 0xffu) == 0) {
+        if (hit1) {
             uint32_t cw = hd;
             for (uint32_t r = m1; r; r &= r - 1u) cw ^= 1u << (S.h(__builtin_ctz(r)) >> 8);
             ss.done = list_hit(ls, e1, cw, ss.pnot, cfg);
@@ -404,7 +511,7 @@ __device__ __forceinline__ void search_run(SearchState &ss, LineList &ls, const 
         ss.T += 1;
         ss.t = t + 2;
         ss.pnot -= e2;
-        if ((syn2 & 0xffu) == 0) {
+        if (hit2) {
             uint32_t cw = hd;
             for (uint32_t r = m2; r; r &= r - 1u) cw ^= 1u << (S.h(__builtin_ctz(r)) >> 8);
             ss.done = list_hit(ls, e2, cw, ss.pnot, cfg);
@@ -412,14 +519,14 @@ __device__ __forceinline__ void search_run(SearchState &ss, LineList &ls, const 
     }
 }
 
-template <bool FINITE_T>
-__device__ __forceinline__ void line_search(const LineLds &S, uint32_t hd, uint32_t s0,
-                                            double pm_hd, int32_t ic, const DecodeCfg &cfg,
-                                            LineList &ls)
+template <bool FINITE_T, class P = PrecF64>
+__device__ __forceinline__ void line_search(const LineLdsT<typename P::Rec> &S, uint32_t hd,
+                                            uint32_t s0, double pm_hd, int32_t ic,
+                                            const DecodeCfg &cfg, LineList &ls)
 {
     SearchState ss;
     search_init(ss, ls, hd, s0, pm_hd, ic, cfg);
-    search_run<FINITE_T>(ss, ls, S, hd, s0, pm_hd, ic, cfg, 0xffffffffu);
+    search_run<FINITE_T, P>(ss, ls, S, hd, s0, pm_hd, ic, cfg, 0xffffffffu);
     ls.T = ss.T;
 }
 

@@ -77,10 +77,11 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v)
 
 constexpr int kPlainThreads = 256;
 
-template <bool FINITE_T>
+template <bool FINITE_T, class P>
 __global__ __launch_bounds__(kPlainThreads) void plain_kernel(CubeJob job)
 {
-    __shared__ double s_delta[17 * kPlainThreads];
+    using Rec = typename P::Rec;
+    __shared__ Rec s_delta[17 * kPlainThreads];
     __shared__ uint16_t s_hp[17 * kPlainThreads];
     __shared__ uint8_t s_hd[kCube];
     __shared__ uint32_t s_sum[kPlainThreads / 64], s_max[kPlainThreads / 64];
@@ -89,7 +90,7 @@ __global__ __launch_bounds__(kPlainThreads) void plain_kernel(CubeJob job)
     const int t = threadIdx.x;
     load_exp_table(s_exp, t);
     __syncthreads();
-    const LineLds S{s_delta, s_hp, kPlainThreads, t};
+    const LineLdsT<Rec> S{s_delta, s_hp, kPlainThreads, t};
     const DecodeCfg cfg{job.tmax, job.thres, job.sched, job.L, s_exp};
     double *le = job.le_scratch + (size_t)blockIdx.x * kCube;
 
@@ -108,8 +109,8 @@ __global__ __launch_bounds__(kPlainThreads) void plain_kernel(CubeJob job)
                 double pm_hd;
                 int32_t ic;
                 LineList ls;
-                if (line_setup(src, S, hd, s0, pm_hd, ic)) {
-                    line_search<FINITE_T>(S, hd, s0, pm_hd, ic, cfg, ls);
+                if (line_setup_p<P>(src, S, hd, s0, pm_hd, ic)) {
+                    line_search<FINITE_T, P>(S, hd, s0, pm_hd, ic, cfg, ls);
                 } else {
                     bad = true;
                     ls.T = 0;
@@ -189,12 +190,13 @@ __global__ __launch_bounds__(kPlainThreads) void plain_kernel(CubeJob job)
     }
 }
 
-template <bool FINITE_T>
+template <bool FINITE_T, class P>
 __global__ __launch_bounds__(256) void lines_kernel(const double *llr, size_t n, double *la,
                                                     double *le, uint32_t *nguess,
                                                     int32_t *status, DecodeCfg cfg)
 {
-    __shared__ double s_delta[17 * 256];
+    using Rec = typename P::Rec;
+    __shared__ Rec s_delta[17 * 256];
     __shared__ uint16_t s_hp[17 * 256];
     __shared__ ExpPair s_exp[32];
     load_exp_table(s_exp, threadIdx.x);
@@ -202,16 +204,16 @@ __global__ __launch_bounds__(256) void lines_kernel(const double *llr, size_t n,
     cfg.exptab = s_exp;
     const size_t line = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (line >= n) return;
-    const LineLds S{s_delta, s_hp, 256, (int)threadIdx.x};
+    const LineLdsT<Rec> S{s_delta, s_hp, 256, (int)threadIdx.x};
     const double *x = llr + line * 16;
     auto src = [x](int i) { return x[i]; };
     uint32_t hd, s0;
     double pm_hd;
     int32_t ic;
     LineList ls;
-    const bool ok = line_setup(src, S, hd, s0, pm_hd, ic);
+    const bool ok = line_setup_p<P>(src, S, hd, s0, pm_hd, ic);
     if (ok) {
-        line_search<FINITE_T>(S, hd, s0, pm_hd, ic, cfg, ls);
+        line_search<FINITE_T, P>(S, hd, s0, pm_hd, ic, cfg, ls);
     } else {
         ls.T = 0;
         ls.nl = 0;
@@ -311,26 +313,34 @@ size_t plain_scratch_slots(size_t n_cw)
 hipError_t launch_plain(const CubeJob &job, int grid, hipStream_t st)
 {
     if (job.n_cw == 0) return hipSuccess;
-    if (job.tmax >= (1ull << 17))
-        hipLaunchKernelGGL(plain_kernel<false>, dim3(grid), dim3(kPlainThreads), 0, st, job);
-    else
-        hipLaunchKernelGGL(plain_kernel<true>, dim3(grid), dim3(kPlainThreads), 0, st, job);
+    const bool finite = job.tmax < (1ull << 17);
+    const dim3 g(grid), b(kPlainThreads);
+    if (job.f32) {
+        if (finite) hipLaunchKernelGGL((plain_kernel<true, PrecF32>), g, b, 0, st, job);
+        else hipLaunchKernelGGL((plain_kernel<false, PrecF32>), g, b, 0, st, job);
+    } else {
+        if (finite) hipLaunchKernelGGL((plain_kernel<true, PrecF64>), g, b, 0, st, job);
+        else hipLaunchKernelGGL((plain_kernel<false, PrecF64>), g, b, 0, st, job);
+    }
     return hipGetLastError();
 }
 
 hipError_t launch_lines(const double *llr, size_t n_lines, double *la, double *le,
                         uint32_t *nguess, int32_t *status, const uint16_t *sched, uint64_t tmax,
-                        double thres, uint32_t L, hipStream_t st)
+                        double thres, uint32_t L, bool f32, hipStream_t st)
 {
     if (n_lines == 0) return hipSuccess;
     const DecodeCfg cfg{tmax, thres, sched, L, nullptr};
-    const unsigned grid = (unsigned)((n_lines + 255) / 256);
-    if (tmax >= (1ull << 17))
-        hipLaunchKernelGGL(lines_kernel<false>, dim3(grid), dim3(256), 0, st, llr, n_lines, la, le,
-                           nguess, status, cfg);
-    else
-        hipLaunchKernelGGL(lines_kernel<true>, dim3(grid), dim3(256), 0, st, llr, n_lines, la, le,
-                           nguess, status, cfg);
+    const dim3 g((unsigned)((n_lines + 255) / 256)), b(256);
+    const bool finite = tmax < (1ull << 17);
+#define SG_LINES(FT, PP)                                                                      \
+    hipLaunchKernelGGL((lines_kernel<FT, PP>), g, b, 0, st, llr, n_lines, la, le, nguess, status, cfg)
+    if (f32) {
+        if (finite) SG_LINES(true, PrecF32); else SG_LINES(false, PrecF32);
+    } else {
+        if (finite) SG_LINES(true, PrecF64); else SG_LINES(false, PrecF64);
+    }
+#undef SG_LINES
     return hipGetLastError();
 }
 

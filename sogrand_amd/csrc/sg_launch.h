@@ -23,6 +23,7 @@ struct CubeJob {
     double alpha;
     uint32_t L;
     uint32_t Imax;
+    bool f32;                 // single-precision path metrics (PrecF32)
 };
 
 // Plain backend: one 256-lane workgroup per cube, one lane per line.
@@ -34,13 +35,13 @@ size_t plain_scratch_slots(size_t n_cw);
 hipError_t launch_persistent(const CubeJob &job, int waves_per_block, int blocks, int slots,
                              uint32_t lane_batch, uint32_t step_len, hipStream_t st);
 // Resident workgroups per CU of the persistent kernel (occupancy API).
-int persistent_blocks_per_cu(int slots, bool finite);
+int persistent_blocks_per_cu(int slots, bool finite, bool f32);
 constexpr int kPersistentWaves = 4;
 
 // Line-level SISO (SOGRAND_bitSO on many lines).
 hipError_t launch_lines(const double *llr, size_t n_lines, double *la, double *le,
                         uint32_t *nguess, int32_t *status, const uint16_t *sched, uint64_t tmax,
-                        double thres, uint32_t L, hipStream_t st);
+                        double thres, uint32_t L, bool f32, hipStream_t st);
 
 // Synthetic LLR generation on the device.
 hipError_t launch_gen(uint64_t seed, uint64_t cw_offset, size_t n_cw, const double *sigma,

@@ -132,31 +132,32 @@ __device__ __forceinline__ double shfl_d(double v, int src)
     return bits_to_double(((uint64_t)hi << 32) | lo);
 }
 
-template <bool FINITE_T, int S>
+template <bool FINITE_T, int S, class P>
 __global__ __launch_bounds__(64 * kWaves) void persistent_kernel(CubeJob job, uint32_t thresh,
                                                                  uint32_t step_len)
 {
-    __shared__ double s_d[kWaves][17 * 64];
+    using Rec = typename P::Rec;
+    __shared__ Rec s_d[kWaves][17 * 64];
     __shared__ uint16_t s_h[kWaves][17 * 64];
     __shared__ uint32_t s_hd[kWaves][S][128];
     __shared__ uint32_t s_cnt[kWaves][S][4]; // lines done, phase guess sum, phase max, bad
     __shared__ double s_stage_jn[kWaves][64];
-    __shared__ uint32_t s_stage_rk[kWaves][64];
+    __shared__ double s_stage_a[kWaves][4][2]; // per sub-round line: |x| of ranks 0 and 7
     __shared__ ExpPair s_exp[32];
 
     load_exp_table(s_exp, threadIdx.x);
     __syncthreads(); // the only workgroup barrier: waves run independently from here
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint64_t lt_mask = (1ull << lane) - 1ull;
-    double *rec_d = &s_d[wave][0];
+    Rec *rec_d = &s_d[wave][0];
     uint16_t *rec_h = &s_h[wave][0];
-    const LineLds L{rec_d, rec_h, 64, lane};
+    const LineLdsT<Rec> L{rec_d, rec_h, 64, lane};
     const DecodeCfg cfg{job.tmax, job.thres, job.sched, job.L, s_exp};
     const size_t wave_id = (size_t)blockIdx.x * kWaves + wave;
     uint32_t *hd_bits = &s_hd[wave][0][0];
     uint32_t *cnt = &s_cnt[wave][0][0];
     double *stage_jn = &s_stage_jn[wave][0];
-    uint32_t *stage_rk = &s_stage_rk[wave][0];
+    double *stage_a = &s_stage_a[wave][0][0];
 
     SlotState<S> sl;
 #pragma unroll
@@ -361,9 +362,10 @@ __global__ __launch_bounds__(64 * kWaves) void persistent_kernel(CubeJob job, ui
                 const int src = owner >= 0 ? owner : 0;
                 const int o_slot = __shfl(my_slot, src, 64);
                 const int o_line = __shfl(my_line, src, 64);
-                const LineLds R{rec_d, rec_h, 64, src}; // the owner's records
-                bool fin = true, tie = false;
+                const LineLdsT<Rec> R{rec_d, rec_h, 64, src}; // the owner's records
+                bool fin = true, tie = false, neg = false;
                 double ai = 0.0, delta = 0.0;
+                Rec key = 0;
                 uint32_t rank = 0;
                 if (owner >= 0) {
                     int base, stride;
@@ -376,58 +378,56 @@ __global__ __launch_bounds__(64 * kWaves) void persistent_kernel(CubeJob job, ui
                     const double tt = job.alpha * e; // L_A = alpha L_E, then + L_channel (:147-155)
                     const double x = lch[p] + tt;
                     fin = __builtin_isfinite(x);
-                    R.d(i) = x;
+                    neg = !(x > 0.0); // HardDec (:758)
                     ai = fabs(x);
+                    key = (Rec)ai;
+                    R.d(i) = key;
                 }
                 if (owner >= 0) {
-                    // rank by counting; ties (equal magnitudes) go to the owner
+                    // rank by counting; in double, equal magnitudes (ties) go to
+                    // the owner, which replays the reference's quicksort; in the
+                    // single-precision variant equal keys are ordered by position
 #pragma unroll 4
                     for (int j = 0; j < 16; ++j) {
-                        const double aj = fabs(R.d(j));
-                        rank += (aj < ai || (aj == ai && j < i)) ? 1u : 0u;
-                        tie = tie || (aj == ai && j != i);
+                        const Rec aj = R.d(j);
+                        rank += (aj < key || (aj == key && j < i)) ? 1u : 0u;
+                        if (!P::kF32) tie = tie || (aj == key && j != i);
                     }
                     const double jn = jaclog(-ai);
                     delta = jaclog(ai) - jn; // getPM's increment (:765)
                     stage_jn[lane] = jn;
-                    stage_rk[lane] = rank;
+                    // the two order statistics of IC do not depend on tie order
+                    if (rank == 0) stage_a[2 * q] = (double)key;
+                    if (rank == 7) stage_a[2 * q + 1] = (double)key;
                 }
                 const uint64_t special = __ballot(owner >= 0 && (tie || !fin));
+                const uint64_t negm = __ballot(owner >= 0 && neg);
+                const uint64_t badm = __ballot(owner >= 0 && !fin);
                 // owners: hard decision, PM_HD in natural order, intercept
                 const int myq = lane == o[0] ? 0 : lane == o[1] ? 1 : lane == o[2] ? 2
                               : lane == o[3] ? 3 : -1;
                 bool line_special = false, line_bad = false;
                 if (myq >= 0) {
                     line_special = ((special >> (16 * myq)) & 0xffffull) != 0;
-                    hd = 0;
-                    s0 = 0;
+                    line_bad = ((badm >> (16 * myq)) & 0xffffull) != 0;
+                    hd = (uint32_t)(negm >> (16 * myq)) & 0xffffu;
+                    s0 = syndrome16(hd);
                     pm_hd = 0.0;
-                    double a_first = 0.0, a_eighth = 0.0;
 #pragma unroll 1
-                    for (int j = 0; j < 16; ++j) {
-                        const double x = L.d(j);
-                        line_bad = line_bad || !__builtin_isfinite(x);
-                        if (!(x > 0.0)) { // HardDec (:758)
-                            hd |= 1u << j;
-                            s0 ^= hcol_of((unsigned)j);
-                        }
-                        pm_hd += stage_jn[16 * myq + j]; // getPM_HD (:617)
-                        const uint32_t r = stage_rk[16 * myq + j];
-                        if (r == 0) a_first = fabs(x);
-                        if (r == 7) a_eighth = fabs(x);
-                    }
+                    for (int j = 0; j < 16; ++j) pm_hd += stage_jn[16 * myq + j]; // getPM_HD (:617)
                     // IC (:619-622)
+                    const double a_first = stage_a[2 * myq], a_eighth = stage_a[2 * myq + 1];
                     const double beta = (a_eighth - a_first) / 7.0;
                     const int32_t rr = cvtt_i32(round(a_first / beta - 1.0));
                     ic = rr > 0 ? rr : 0;
                 }
                 // tasks of ordinary lines write the sorted records
                 if (owner >= 0 && ((special >> (16 * q)) & 0xffffull) == 0) {
-                    R.d((int)rank) = delta;
+                    R.d((int)rank) = (Rec)delta;
                     R.h((int)rank) = (uint16_t)(hcol_of((unsigned)i) | (i << 8));
                 }
                 if (myq >= 0) {
-                    L.d(16) = 0.0; // null record read by exhausted masks in search_run
+                    L.d(16) = (Rec)0; // null record read by exhausted masks in search_run
                     L.h(16) = 0;
                     if (line_bad) {
                         atomicOr(&cnt[4 * my_slot + 3], 1u);
@@ -437,10 +437,9 @@ __global__ __launch_bounds__(64 * kWaves) void persistent_kernel(CubeJob job, ui
                         ss.done = true;
                         state = kDone;
                     } else {
-                        if (line_special) {
-#pragma unroll 1
-                            for (int j = 0; j < 16; ++j) L.d(j) = fabs(L.d(j));
-                            pm_hd = setup_with_ties(L);
+                        if constexpr (!P::kF32) {
+                            // records hold |x| in natural order
+                            if (line_special) pm_hd = setup_with_ties(L);
                         }
                         search_init(ss, ls, hd, s0, pm_hd, ic, cfg);
                         state = ss.done ? kDone : kSearch;
@@ -450,23 +449,27 @@ __global__ __launch_bounds__(64 * kWaves) void persistent_kernel(CubeJob job, ui
         }
         // ---- 4. a bounded step of every searching lane's guessing loop ----
         if (state == kSearch) {
-            search_run<FINITE_T>(ss, ls, L, hd, s0, pm_hd, ic, cfg, step_len);
+            search_run<FINITE_T, P>(ss, ls, L, hd, s0, pm_hd, ic, cfg, step_len);
             if (ss.done) state = kDone;
         }
     }
 }
 
-int persistent_blocks_per_cu(int slots, bool finite)
+int persistent_blocks_per_cu(int slots, bool finite, bool f32)
 {
     int nb = 0;
     hipError_t e = hipErrorInvalidValue;
-#define SG_OCC(FT, SS)                                                                        \
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, persistent_kernel<FT, SS>, 64 * kWaves, 0)
+#define SG_OCC(FT, SS, PP)                                                                    \
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, persistent_kernel<FT, SS, PP>,       \
+                                                     64 * kWaves, 0)
+#define SG_OCC_P(FT, SS)                                                                      \
+    if (f32) SG_OCC(FT, SS, PrecF32); else SG_OCC(FT, SS, PrecF64)
     if (slots == 1) {
-        if (finite) SG_OCC(true, 1); else SG_OCC(false, 1);
+        if (finite) { SG_OCC_P(true, 1); } else { SG_OCC_P(false, 1); }
     } else if (slots == 2) {
-        if (finite) SG_OCC(true, 2); else SG_OCC(false, 2);
+        if (finite) { SG_OCC_P(true, 2); } else { SG_OCC_P(false, 2); }
     }
+#undef SG_OCC_P
 #undef SG_OCC
     if (e != hipSuccess) {
         (void)hipGetLastError();
@@ -483,15 +486,18 @@ hipError_t launch_persistent(const CubeJob &job, int waves_per_block, int blocks
     const bool finite = job.tmax < (1ull << 17);
     const uint32_t thresh = lane_batch, step = step_len;
     const dim3 g(blocks), b(64 * kWaves);
-#define SG_LAUNCH(FT, SS)                                                                     \
-    hipLaunchKernelGGL((persistent_kernel<FT, SS>), g, b, 0, st, job, thresh, step)
+#define SG_LAUNCH(FT, SS, PP)                                                                 \
+    hipLaunchKernelGGL((persistent_kernel<FT, SS, PP>), g, b, 0, st, job, thresh, step)
+#define SG_LAUNCH_P(FT, SS)                                                                   \
+    if (job.f32) SG_LAUNCH(FT, SS, PrecF32); else SG_LAUNCH(FT, SS, PrecF64)
     if (slots == 1) {
-        if (finite) SG_LAUNCH(true, 1); else SG_LAUNCH(false, 1);
+        if (finite) { SG_LAUNCH_P(true, 1); } else { SG_LAUNCH_P(false, 1); }
     } else if (slots == 2) {
-        if (finite) SG_LAUNCH(true, 2); else SG_LAUNCH(false, 2);
+        if (finite) { SG_LAUNCH_P(true, 2); } else { SG_LAUNCH_P(false, 2); }
     } else {
         return hipErrorInvalidValue;
     }
+#undef SG_LAUNCH_P
 #undef SG_LAUNCH
     return hipGetLastError();
 }

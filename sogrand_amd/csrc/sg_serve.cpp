@@ -1,0 +1,360 @@
+// Serving loop: continuous batching of decode requests (frames of codewords).
+//
+// A worker thread owns `max_inflight` batch slots. Each slot has its own
+// non-blocking HIP stream and a fixed carve of the device-memory budget (LLRs,
+// L_E scratch, bits, statistics, status of up to `max_batch_cw` codewords), so
+// admission never allocates. Requests wait in a FIFO; whenever a slot is free,
+// every waiting request that fits is gathered into one batch (input copies on
+// the slot's stream), decoded by one launch of the configured kernel, and the
+// results are copied back when the slot's completion event fires. Batches on
+// different slots run concurrently on the device: a request that arrives
+// while others decode is admitted as soon as a slot frees up, and its
+// workgroups take compute units as the earlier launches drain.
+//
+// Each codeword's result depends only on its LLRs (the decoder is batch
+// independent), so a request decodes to the same bits, statistics and soft
+// output alone or in any mix.
+//
+// The reference has no serving path; its nearest relative is the multi-stream
+// CUDA decoder (SOGRAND_CUDA/cubic_decoder.cu:1145-1179), which writes batches
+// out of order. Here results are returned by request id.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <new>
+#include <thread>
+#include <vector>
+
+#include "sg_common.h"
+#include "sg_ctx.h"
+#include "sg_launch.h"
+#include "sogrand.h"
+
+namespace {
+
+struct Request {
+    uint64_t id;
+    const double *llr;
+    bool llr_dev;
+    size_t n;
+    uint8_t *info;
+    double *soft;
+    sogrand_cw_stats *stats;
+    double t_submit;
+};
+
+struct Slot {
+    hipStream_t st = nullptr;
+    hipEvent_t done = nullptr;
+    double *llr = nullptr, *le = nullptr, *soft = nullptr;
+    uint8_t *info = nullptr;
+    uint32_t *stats = nullptr, *queue = nullptr;
+    int32_t *status = nullptr;
+    void *base = nullptr; // one allocation per slot
+    bool busy = false;
+    double t_start = 0.0;
+    std::vector<Request> reqs;
+    std::vector<int32_t> h_status;
+};
+
+} // namespace
+
+struct sogrand_server {
+    sogrand_ctx *ctx = nullptr;
+    sogrand_params p{};
+    sogrand_server_cfg cfg{};
+    size_t batch_cw = 0;
+    std::vector<Slot> slots;
+    std::deque<Request> pending;
+    size_t pending_cw = 0;
+    std::deque<sogrand_completion> completed;
+    uint64_t submitted = 0, finished = 0;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    bool stop = false;
+    bool fatal = false;
+    std::thread worker;
+    std::chrono::steady_clock::time_point t0;
+
+    double now() const
+    {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    void run();
+    bool dispatch(Slot &s);
+    void finish(Slot &s);
+};
+
+// Gather every waiting request that fits into slot s and launch its batch.
+bool sogrand_server::dispatch(Slot &s)
+{
+    s.reqs.clear();
+    size_t n = 0;
+    while (!pending.empty() && n + pending.front().n <= batch_cw) {
+        s.reqs.push_back(pending.front());
+        n += pending.front().n;
+        pending_cw -= pending.front().n;
+        pending.pop_front();
+    }
+    if (s.reqs.empty()) return false;
+    bool want_soft = false;
+    size_t off = 0;
+    for (const Request &r : s.reqs) {
+        const hipMemcpyKind k = r.llr_dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+        if (hipMemcpyAsync(s.llr + off * sg::kCube, r.llr, r.n * sg::kCube * sizeof(double), k,
+                           s.st) != hipSuccess)
+            fatal = true;
+        want_soft = want_soft || r.soft;
+        off += r.n;
+    }
+    sg::CubeJob job{};
+    job.llr = s.llr;
+    job.info = s.info;
+    job.soft = want_soft ? s.soft : nullptr;
+    job.stats = s.stats;
+    job.status = s.status;
+    job.le_scratch = s.le;
+    job.queue = s.queue;
+    job.n_cw = n;
+    sg::fill_job(job, ctx, p);
+    const sg::LaunchPlan lp = sg::plan_launch(ctx, p, n);
+    if (sg::launch_cubes(p, lp, job, s.st, nullptr) != hipSuccess) fatal = true;
+    if (hipEventRecord(s.done, s.st) != hipSuccess) fatal = true;
+    s.busy = true;
+    s.t_start = now();
+    return true;
+}
+
+// Copy a finished batch's results to its requests and report them.
+void sogrand_server::finish(Slot &s)
+{
+    size_t n = 0;
+    for (const Request &r : s.reqs) n += r.n;
+    s.h_status.resize(n);
+    bool ok = hipMemcpyAsync(s.h_status.data(), s.status, n * sizeof(int32_t),
+                             hipMemcpyDeviceToHost, s.st) == hipSuccess;
+    size_t off = 0;
+    for (const Request &r : s.reqs) {
+        ok = ok && hipMemcpyAsync(r.info, s.info + off * sg::kInfoBytes, r.n * sg::kInfoBytes,
+                                  hipMemcpyDefault, s.st) == hipSuccess;
+        if (r.stats)
+            ok = ok && hipMemcpyAsync(r.stats, s.stats + off * 6, r.n * sizeof(sogrand_cw_stats),
+                                      hipMemcpyDefault, s.st) == hipSuccess;
+        if (r.soft)
+            ok = ok && hipMemcpyAsync(r.soft, s.soft + off * sg::kCube,
+                                      r.n * sg::kCube * sizeof(double), hipMemcpyDefault,
+                                      s.st) == hipSuccess;
+        off += r.n;
+    }
+    ok = ok && hipStreamSynchronize(s.st) == hipSuccess;
+    const double t = now();
+    off = 0;
+    for (const Request &r : s.reqs) {
+        int32_t st = ok ? SOGRAND_OK : SOGRAND_EHIP;
+        for (size_t i = 0; ok && i < r.n; ++i)
+            if (s.h_status[off + i]) st = SOGRAND_ENONFINITE;
+        completed.push_back(sogrand_completion{r.id, st, (uint32_t)r.n, r.t_submit, s.t_start, t});
+        off += r.n;
+        ++finished;
+    }
+    if (!ok) fatal = true;
+    s.reqs.clear();
+    s.busy = false;
+}
+
+void sogrand_server::run()
+{
+    (void)hipSetDevice(ctx->device);
+    std::unique_lock<std::mutex> lk(mu);
+    for (;;) {
+        bool progress = false;
+        for (Slot &s : slots) {
+            if (!s.busy) continue;
+            const hipError_t q = hipEventQuery(s.done);
+            if (q == hipErrorNotReady) continue;
+            if (q != hipSuccess) fatal = true;
+            finish(s);
+            progress = true;
+        }
+        for (Slot &s : slots) {
+            if (s.busy || pending.empty()) continue;
+            progress = dispatch(s) || progress;
+        }
+        if (progress) cv_done.notify_all();
+        bool busy = false;
+        for (const Slot &s : slots) busy = busy || s.busy;
+        if (stop && !busy && pending.empty()) break;
+        if (!progress) {
+            // poll in-flight batches every 20 us; sleep until new work otherwise
+            if (busy)
+                cv_work.wait_for(lk, std::chrono::microseconds(20));
+            else
+                cv_work.wait(lk, [&] { return stop || !pending.empty(); });
+        }
+    }
+}
+
+extern "C" {
+
+void sogrand_server_cfg_default(sogrand_server_cfg *c)
+{
+    if (!c) return;
+    std::memset(c, 0, sizeof *c);
+    c->hbm_budget = (size_t)4 << 30;
+    c->max_inflight = 4;
+    c->max_batch_cw = 0;
+    c->max_pending_cw = 1u << 20;
+    c->with_soft = 0;
+}
+
+int sogrand_server_create(sogrand_server **out, sogrand_ctx *ctx, const sogrand_server_cfg *cfg)
+{
+    if (!out || !ctx) return SOGRAND_EINVAL;
+    *out = nullptr;
+    sogrand_server_cfg c;
+    if (cfg)
+        c = *cfg;
+    else
+        sogrand_server_cfg_default(&c);
+    if (c.max_inflight < 1 || c.max_inflight > 16) return SOGRAND_EINVAL;
+    sogrand_server *s = new (std::nothrow) sogrand_server;
+    if (!s) return SOGRAND_ENOMEM;
+    s->ctx = ctx;
+    {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        s->p = ctx->p;
+    }
+    s->cfg = c;
+    // per-codeword bytes of a slot: LLRs, L_E scratch, optional soft output,
+    // bits, statistics, status
+    const size_t per_cw = 2 * sg::kCube * sizeof(double) +
+                          (c.with_soft ? sg::kCube * sizeof(double) : 0) + sg::kInfoBytes +
+                          sizeof(sogrand_cw_stats) + sizeof(int32_t);
+    size_t b = c.hbm_budget / c.max_inflight / per_cw;
+    if (b > 65536) b = 65536;
+    if (c.max_batch_cw && b > c.max_batch_cw) b = c.max_batch_cw;
+    if (b < 1) {
+        delete s;
+        return SOGRAND_EINVAL; // budget too small for one codeword per slot
+    }
+    s->batch_cw = b;
+    if (hipSetDevice(ctx->device) != hipSuccess) {
+        delete s;
+        return SOGRAND_EHIP;
+    }
+    s->slots.resize(c.max_inflight);
+    for (Slot &sl : s->slots) {
+        const size_t le_n = sg::max_le_slots(b) * sg::kCube;
+        const size_t bytes_d = (b * sg::kCube + le_n + (c.with_soft ? b * sg::kCube : 0)) *
+                               sizeof(double);
+        const size_t bytes = bytes_d + b * 6 * sizeof(uint32_t) + b * sizeof(int32_t) + 256 +
+                             b * sg::kInfoBytes;
+        if (hipMalloc(&sl.base, bytes) != hipSuccess) {
+            sl.base = nullptr;
+            sogrand_server_destroy(s);
+            return SOGRAND_ENOMEM;
+        }
+        char *q = (char *)sl.base;
+        sl.llr = (double *)q;
+        sl.le = sl.llr + b * sg::kCube;
+        sl.soft = c.with_soft ? sl.le + le_n : nullptr;
+        q += bytes_d;
+        sl.stats = (uint32_t *)q;
+        q += b * 6 * sizeof(uint32_t);
+        sl.status = (int32_t *)q;
+        q += b * sizeof(int32_t);
+        sl.queue = (uint32_t *)q;
+        q += 256;
+        sl.info = (uint8_t *)q;
+        if (hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess) {
+            sogrand_server_destroy(s);
+            return SOGRAND_EHIP;
+        }
+    }
+    s->t0 = std::chrono::steady_clock::now();
+    s->worker = std::thread([s] { s->run(); });
+    *out = s;
+    return SOGRAND_OK;
+}
+
+int sogrand_server_batch_cw(const sogrand_server *s, size_t *batch_cw)
+{
+    if (!s || !batch_cw) return SOGRAND_EINVAL;
+    *batch_cw = s->batch_cw;
+    return SOGRAND_OK;
+}
+
+int sogrand_server_submit(sogrand_server *s, uint64_t id, const double *llr, size_t n_cw,
+                          uint8_t *info_bits, double *soft_out, sogrand_cw_stats *stats)
+{
+    if (!s || !llr || !info_bits || n_cw == 0) return SOGRAND_EINVAL;
+    if (n_cw > s->batch_cw) return SOGRAND_EINVAL; // larger than one batch slot
+    if (soft_out && !s->cfg.with_soft) return SOGRAND_EINVAL;
+    Request r{id, llr, sg::is_device_ptr(llr), n_cw, info_bits, soft_out, stats, 0.0};
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        if (s->fatal) return SOGRAND_EHIP;
+        if (s->pending_cw + n_cw > s->cfg.max_pending_cw) return SOGRAND_EBUSY;
+        r.t_submit = s->now();
+        s->pending.push_back(r);
+        s->pending_cw += n_cw;
+        ++s->submitted;
+    }
+    s->cv_work.notify_one();
+    return SOGRAND_OK;
+}
+
+int sogrand_server_poll(sogrand_server *s, sogrand_completion *out, size_t max, size_t *n_out,
+                        int wait_ms)
+{
+    if (!s || !n_out || (!out && max)) return SOGRAND_EINVAL;
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (s->completed.empty() && wait_ms > 0)
+        s->cv_done.wait_for(lk, std::chrono::milliseconds(wait_ms),
+                            [&] { return !s->completed.empty() || s->fatal; });
+    size_t k = 0;
+    while (k < max && !s->completed.empty()) {
+        out[k++] = s->completed.front();
+        s->completed.pop_front();
+    }
+    *n_out = k;
+    return s->fatal ? SOGRAND_EHIP : SOGRAND_OK;
+}
+
+int sogrand_server_drain(sogrand_server *s)
+{
+    if (!s) return SOGRAND_EINVAL;
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->cv_done.wait(lk, [&] { return s->finished == s->submitted || s->fatal; });
+    return s->fatal ? SOGRAND_EHIP : SOGRAND_OK;
+}
+
+double sogrand_server_clock(const sogrand_server *s) { return s ? s->now() : 0.0; }
+
+void sogrand_server_destroy(sogrand_server *s)
+{
+    if (!s) return;
+    if (s->worker.joinable()) {
+        {
+            std::lock_guard<std::mutex> g(s->mu);
+            s->stop = true;
+        }
+        s->cv_work.notify_all();
+        s->worker.join();
+    }
+    (void)hipSetDevice(s->ctx->device);
+    for (Slot &sl : s->slots) {
+        if (sl.st) (void)hipStreamSynchronize(sl.st);
+        if (sl.base) (void)hipFree(sl.base);
+        if (sl.done) (void)hipEventDestroy(sl.done);
+        if (sl.st) (void)hipStreamDestroy(sl.st);
+    }
+    delete s;
+}
+
+} // extern "C"
