@@ -158,10 +158,15 @@ typedef struct sogrand_server sogrand_server;
 
 typedef struct sogrand_server_cfg {
     size_t hbm_budget;       /* device bytes for all batch slots (default 4 GiB)          */
-    uint32_t max_inflight;   /* batch slots = concurrent launches, 1..16 (default 4)       */
+    uint32_t max_inflight;   /* batch slots = concurrent launches, 1..16 (default 16; they
+                                run concurrently up to the HIP runtime's hardware queues,
+                                GPU_MAX_HW_QUEUES)                                          */
     uint32_t max_batch_cw;   /* codewords per batch (0 = what the budget allows, <= 65536) */
     uint32_t max_pending_cw; /* waiting codewords before submit refuses (default 2^20)     */
     int32_t with_soft;       /* reserve room for soft outputs (default 0)                  */
+    uint32_t plain_below_cw; /* batches with fewer codewords run the plain kernel, which
+                                finishes a single pass over the GPU sooner (0 = 3 per CU;
+                                1 = always the configured backend)                        */
 } sogrand_server_cfg;
 
 typedef struct sogrand_completion {

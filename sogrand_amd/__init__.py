@@ -82,6 +82,7 @@ class ServerCfg(C.Structure):
         ("max_batch_cw", C.c_uint32),
         ("max_pending_cw", C.c_uint32),
         ("with_soft", C.c_int32),
+        ("plain_below_cw", C.c_uint32),
     ]
 
 
@@ -277,8 +278,9 @@ class Server:
     are filled before the frame's completion is reported; the caller keeps the
     input and outputs alive until then."""
 
-    def __init__(self, decoder: Decoder, hbm_budget: int | None = None, max_inflight: int = 4,
-                 max_batch_cw: int = 0, max_pending_cw: int = 1 << 20, with_soft: bool = False):
+    def __init__(self, decoder: Decoder, hbm_budget: int | None = None, max_inflight: int = 16,
+                 max_batch_cw: int = 0, max_pending_cw: int = 1 << 20, with_soft: bool = False,
+                 plain_below_cw: int = 0):
         lib = decoder._lib
         self._lib = lib
         cfg = ServerCfg()
@@ -289,6 +291,7 @@ class Server:
         cfg.max_batch_cw = max_batch_cw
         cfg.max_pending_cw = max_pending_cw
         cfg.with_soft = 1 if with_soft else 0
+        cfg.plain_below_cw = plain_below_cw
         h = C.c_void_p()
         _check(lib.sogrand_server_create(C.byref(h), decoder._h, C.byref(cfg)), "server_create")
         self._h = h

@@ -49,7 +49,7 @@ LaunchPlan plan_launch(const sogrand_ctx *ctx, const sogrand_params &p, size_t n
         // a persistent grid must be resident all at once: never more workgroups
         // per CU than the occupancy the kernel's registers and LDS allow
         const uint64_t tmax = effective_tmax(p.Tmax);
-        const int occ = persistent_blocks_per_cu(slots, tmax < (1ull << 17),
+        const int occ = persistent_blocks_per_cu(slots, tmax_binds(tmax),
                                                  p.precision == SOGRAND_FP32_MIXED);
         int bpc = p.waves_per_cu > 0 ? (p.waves_per_cu + wpb - 1) / wpb : occ;
         if (bpc > occ) bpc = occ;

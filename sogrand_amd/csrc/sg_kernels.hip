@@ -313,7 +313,7 @@ size_t plain_scratch_slots(size_t n_cw)
 hipError_t launch_plain(const CubeJob &job, int grid, hipStream_t st)
 {
     if (job.n_cw == 0) return hipSuccess;
-    const bool finite = job.tmax < (1ull << 17);
+    const bool finite = tmax_binds(job.tmax);
     const dim3 g(grid), b(kPlainThreads);
     if (job.f32) {
         if (finite) hipLaunchKernelGGL((plain_kernel<true, PrecF32>), g, b, 0, st, job);
@@ -332,7 +332,7 @@ hipError_t launch_lines(const double *llr, size_t n_lines, double *la, double *l
     if (n_lines == 0) return hipSuccess;
     const DecodeCfg cfg{tmax, thres, sched, L, nullptr};
     const dim3 g((unsigned)((n_lines + 255) / 256)), b(256);
-    const bool finite = tmax < (1ull << 17);
+    const bool finite = tmax_binds(tmax);
 #define SG_LINES(FT, PP)                                                                      \
     hipLaunchKernelGGL((lines_kernel<FT, PP>), g, b, 0, st, llr, n_lines, la, le, nguess, status, cfg)
     if (f32) {

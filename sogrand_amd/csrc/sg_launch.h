@@ -26,6 +26,12 @@ struct CubeJob {
     bool f32;                 // single-precision path metrics (PrecF32)
 };
 
+// Whether a guess budget can stop a search. A line tests at most 65536 patterns
+// (the hard decision and 65535 others), so the reference's guard T < Tmax,
+// taken before each weight level with T <= 65535 (cubic_decoder_final.c:652),
+// can only fail for Tmax <= 65535; larger budgets run the unbounded kernels.
+inline bool tmax_binds(uint64_t tmax) { return tmax < 65536u; }
+
 // Plain backend: one 256-lane workgroup per cube, one lane per line.
 hipError_t launch_plain(const CubeJob &job, int grid, hipStream_t st);
 size_t plain_scratch_slots(size_t n_cw);

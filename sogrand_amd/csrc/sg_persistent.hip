@@ -483,7 +483,7 @@ hipError_t launch_persistent(const CubeJob &job, int waves_per_block, int blocks
 {
     if (job.n_cw == 0) return hipSuccess;
     if (waves_per_block != kWaves) return hipErrorInvalidValue;
-    const bool finite = job.tmax < (1ull << 17);
+    const bool finite = tmax_binds(job.tmax);
     const uint32_t thresh = lane_batch, step = step_len;
     const dim3 g(blocks), b(64 * kWaves);
 #define SG_LAUNCH(FT, SS, PP)                                                                 \

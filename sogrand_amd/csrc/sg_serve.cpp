@@ -39,12 +39,12 @@ namespace {
 struct Request {
     uint64_t id;
     const double *llr;
-    bool llr_dev;
     size_t n;
     uint8_t *info;
     double *soft;
     sogrand_cw_stats *stats;
     double t_submit;
+    bool llr_dev, info_dev, soft_dev, stats_dev; // device (not host) memory
 };
 
 struct Slot {
@@ -54,11 +54,15 @@ struct Slot {
     uint8_t *info = nullptr;
     uint32_t *stats = nullptr, *queue = nullptr;
     int32_t *status = nullptr;
-    void *base = nullptr; // one allocation per slot
+    void *base = nullptr; // one device allocation per slot
+    // pinned host staging: results (info | stats | status) always, soft output
+    // when reserved, host-resident inputs on first use
+    uint8_t *h_out = nullptr;
+    double *h_soft = nullptr;
+    double *h_llr = nullptr;
     bool busy = false;
     double t_start = 0.0;
     std::vector<Request> reqs;
-    std::vector<int32_t> h_status;
 };
 
 } // namespace
@@ -68,6 +72,7 @@ struct sogrand_server {
     sogrand_params p{};
     sogrand_server_cfg cfg{};
     size_t batch_cw = 0;
+    size_t plain_below = 0; // batches smaller than this run the plain kernel
     std::vector<Slot> slots;
     std::deque<Request> pending;
     size_t pending_cw = 0;
@@ -85,12 +90,13 @@ struct sogrand_server {
         return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
     void run();
-    bool dispatch(Slot &s);
-    void finish(Slot &s);
+    void take(Slot &s);
+    void launch(Slot &s);
+    void finish(Slot &s, bool event_ok);
 };
 
-// Gather every waiting request that fits into slot s and launch its batch.
-bool sogrand_server::dispatch(Slot &s)
+// Move every waiting request that fits into slot s (caller holds the lock).
+void sogrand_server::take(Slot &s)
 {
     s.reqs.clear();
     size_t n = 0;
@@ -100,15 +106,38 @@ bool sogrand_server::dispatch(Slot &s)
         pending_cw -= pending.front().n;
         pending.pop_front();
     }
-    if (s.reqs.empty()) return false;
-    bool want_soft = false;
+}
+
+// Enqueue the batch of slot s: inputs, decode, results to pinned staging (or
+// straight to device-resident outputs), completion event. Worker thread only,
+// without the lock.
+void sogrand_server::launch(Slot &s)
+{
+    bool ok = true, want_soft = false, host_in = false;
+    size_t n = 0;
+    for (const Request &r : s.reqs) {
+        want_soft = want_soft || r.soft;
+        host_in = host_in || !r.llr_dev;
+        n += r.n;
+    }
+    if (host_in && !s.h_llr &&
+        hipHostMalloc((void **)&s.h_llr, batch_cw * sg::kCube * sizeof(double),
+                      hipHostMallocDefault) != hipSuccess) {
+        s.h_llr = nullptr;
+        ok = false;
+    }
     size_t off = 0;
     for (const Request &r : s.reqs) {
-        const hipMemcpyKind k = r.llr_dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
-        if (hipMemcpyAsync(s.llr + off * sg::kCube, r.llr, r.n * sg::kCube * sizeof(double), k,
-                           s.st) != hipSuccess)
-            fatal = true;
-        want_soft = want_soft || r.soft;
+        const size_t bytes = r.n * sg::kCube * sizeof(double);
+        double *dst = s.llr + off * sg::kCube;
+        if (r.llr_dev) {
+            ok = ok && hipMemcpyAsync(dst, r.llr, bytes, hipMemcpyDeviceToDevice, s.st) ==
+                           hipSuccess;
+        } else if (ok) {
+            std::memcpy(s.h_llr + off * sg::kCube, r.llr, bytes);
+            ok = hipMemcpyAsync(dst, s.h_llr + off * sg::kCube, bytes, hipMemcpyHostToDevice,
+                                s.st) == hipSuccess;
+        }
         off += r.n;
     }
     sg::CubeJob job{};
@@ -121,72 +150,108 @@ bool sogrand_server::dispatch(Slot &s)
     job.queue = s.queue;
     job.n_cw = n;
     sg::fill_job(job, ctx, p);
-    const sg::LaunchPlan lp = sg::plan_launch(ctx, p, n);
-    if (sg::launch_cubes(p, lp, job, s.st, nullptr) != hipSuccess) fatal = true;
-    if (hipEventRecord(s.done, s.st) != hipSuccess) fatal = true;
-    s.busy = true;
-    s.t_start = now();
-    return true;
-}
-
-// Copy a finished batch's results to its requests and report them.
-void sogrand_server::finish(Slot &s)
-{
-    size_t n = 0;
-    for (const Request &r : s.reqs) n += r.n;
-    s.h_status.resize(n);
-    bool ok = hipMemcpyAsync(s.h_status.data(), s.status, n * sizeof(int32_t),
-                             hipMemcpyDeviceToHost, s.st) == hipSuccess;
-    size_t off = 0;
-    for (const Request &r : s.reqs) {
-        ok = ok && hipMemcpyAsync(r.info, s.info + off * sg::kInfoBytes, r.n * sg::kInfoBytes,
-                                  hipMemcpyDefault, s.st) == hipSuccess;
-        if (r.stats)
-            ok = ok && hipMemcpyAsync(r.stats, s.stats + off * 6, r.n * sizeof(sogrand_cw_stats),
-                                      hipMemcpyDefault, s.st) == hipSuccess;
-        if (r.soft)
-            ok = ok && hipMemcpyAsync(r.soft, s.soft + off * sg::kCube,
-                                      r.n * sg::kCube * sizeof(double), hipMemcpyDefault,
-                                      s.st) == hipSuccess;
-        off += r.n;
-    }
-    ok = ok && hipStreamSynchronize(s.st) == hipSuccess;
-    const double t = now();
+    sogrand_params q = p;
+    if (n < plain_below) q.backend = SOGRAND_BACKEND_PLAIN; // one pass over the GPU: less latency
+    const sg::LaunchPlan lp = sg::plan_launch(ctx, q, n);
+    ok = ok && sg::launch_cubes(q, lp, job, s.st, nullptr) == hipSuccess;
+    // results: device-resident outputs directly, host outputs via pinned staging
+    uint8_t *h_info = s.h_out;
+    uint32_t *h_stats = (uint32_t *)(s.h_out + batch_cw * sg::kInfoBytes);
+    int32_t *h_status = (int32_t *)(h_stats + batch_cw * 6);
+    ok = ok && hipMemcpyAsync(h_status, s.status, n * sizeof(int32_t), hipMemcpyDeviceToHost,
+                              s.st) == hipSuccess;
     off = 0;
     for (const Request &r : s.reqs) {
-        int32_t st = ok ? SOGRAND_OK : SOGRAND_EHIP;
-        for (size_t i = 0; ok && i < r.n; ++i)
-            if (s.h_status[off + i]) st = SOGRAND_ENONFINITE;
-        completed.push_back(sogrand_completion{r.id, st, (uint32_t)r.n, r.t_submit, s.t_start, t});
+        const bool dev_info = r.info_dev;
+        ok = ok && hipMemcpyAsync(dev_info ? (void *)r.info : (void *)(h_info + off * sg::kInfoBytes),
+                                  s.info + off * sg::kInfoBytes, r.n * sg::kInfoBytes,
+                                  dev_info ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                                  s.st) == hipSuccess;
+        if (r.stats) {
+            const bool dev = r.stats_dev;
+            ok = ok && hipMemcpyAsync(dev ? (void *)r.stats : (void *)(h_stats + off * 6),
+                                      s.stats + off * 6, r.n * sizeof(sogrand_cw_stats),
+                                      dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                                      s.st) == hipSuccess;
+        }
+        if (r.soft) {
+            const bool dev = r.soft_dev;
+            ok = ok && hipMemcpyAsync(dev ? (void *)r.soft : (void *)(s.h_soft + off * sg::kCube),
+                                      s.soft + off * sg::kCube, r.n * sg::kCube * sizeof(double),
+                                      dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                                      s.st) == hipSuccess;
+        }
         off += r.n;
-        ++finished;
     }
-    if (!ok) fatal = true;
+    ok = ok && hipEventRecord(s.done, s.st) == hipSuccess;
+    if (!ok) {
+        std::lock_guard<std::mutex> g(mu);
+        fatal = true;
+    }
+}
+
+// Results of a finished batch to the requests' host buffers, then report them.
+// Worker thread only; takes the lock only to publish.
+void sogrand_server::finish(Slot &s, bool event_ok)
+{
+    const uint8_t *h_info = s.h_out;
+    const uint32_t *h_stats = (const uint32_t *)(s.h_out + batch_cw * sg::kInfoBytes);
+    const int32_t *h_status = (const int32_t *)(h_stats + batch_cw * 6);
+    size_t off = 0;
+    std::vector<sogrand_completion> out;
+    out.reserve(s.reqs.size());
+    const double t = now();
+    for (const Request &r : s.reqs) {
+        if (event_ok) {
+            if (!r.info_dev)
+                std::memcpy(r.info, h_info + off * sg::kInfoBytes, r.n * sg::kInfoBytes);
+            if (r.stats && !r.stats_dev)
+                std::memcpy(r.stats, h_stats + off * 6, r.n * sizeof(sogrand_cw_stats));
+            if (r.soft && !r.soft_dev)
+                std::memcpy(r.soft, s.h_soft + off * sg::kCube, r.n * sg::kCube * sizeof(double));
+        }
+        int32_t st = event_ok ? SOGRAND_OK : SOGRAND_EHIP;
+        for (size_t i = 0; event_ok && i < r.n; ++i)
+            if (h_status[off + i]) st = SOGRAND_ENONFINITE;
+        out.push_back(sogrand_completion{r.id, st, (uint32_t)r.n, r.t_submit, s.t_start, t});
+        off += r.n;
+    }
     s.reqs.clear();
     s.busy = false;
+    std::lock_guard<std::mutex> g(mu);
+    for (const sogrand_completion &c : out) completed.push_back(c);
+    finished += out.size();
+    if (!event_ok) fatal = true;
 }
 
 void sogrand_server::run()
 {
     (void)hipSetDevice(ctx->device);
-    std::unique_lock<std::mutex> lk(mu);
     for (;;) {
         bool progress = false;
         for (Slot &s : slots) {
             if (!s.busy) continue;
             const hipError_t q = hipEventQuery(s.done);
             if (q == hipErrorNotReady) continue;
-            if (q != hipSuccess) fatal = true;
-            finish(s);
+            finish(s, q == hipSuccess);
             progress = true;
         }
         for (Slot &s : slots) {
-            if (s.busy || pending.empty()) continue;
-            progress = dispatch(s) || progress;
+            if (s.busy) continue;
+            {
+                std::lock_guard<std::mutex> g(mu);
+                take(s);
+            }
+            if (s.reqs.empty()) break;
+            s.busy = true;
+            s.t_start = now();
+            launch(s);
+            progress = true;
         }
         if (progress) cv_done.notify_all();
         bool busy = false;
         for (const Slot &s : slots) busy = busy || s.busy;
+        std::unique_lock<std::mutex> lk(mu);
         if (stop && !busy && pending.empty()) break;
         if (!progress) {
             // poll in-flight batches every 20 us; sleep until new work otherwise
@@ -205,10 +270,11 @@ void sogrand_server_cfg_default(sogrand_server_cfg *c)
     if (!c) return;
     std::memset(c, 0, sizeof *c);
     c->hbm_budget = (size_t)4 << 30;
-    c->max_inflight = 4;
+    c->max_inflight = 16;
     c->max_batch_cw = 0;
     c->max_pending_cw = 1u << 20;
     c->with_soft = 0;
+    c->plain_below_cw = 0;
 }
 
 int sogrand_server_create(sogrand_server **out, sogrand_ctx *ctx, const sogrand_server_cfg *cfg)
@@ -242,6 +308,7 @@ int sogrand_server_create(sogrand_server **out, sogrand_ctx *ctx, const sogrand_
         return SOGRAND_EINVAL; // budget too small for one codeword per slot
     }
     s->batch_cw = b;
+    s->plain_below = c.plain_below_cw ? c.plain_below_cw : (size_t)3 * ctx->cu_count;
     if (hipSetDevice(ctx->device) != hipSuccess) {
         delete s;
         return SOGRAND_EHIP;
@@ -270,6 +337,13 @@ int sogrand_server_create(sogrand_server **out, sogrand_ctx *ctx, const sogrand_
         sl.queue = (uint32_t *)q;
         q += 256;
         sl.info = (uint8_t *)q;
+        const size_t out_bytes = b * (sg::kInfoBytes + 6 * sizeof(uint32_t) + sizeof(int32_t));
+        if (hipHostMalloc((void **)&sl.h_out, out_bytes, hipHostMallocDefault) != hipSuccess ||
+            (c.with_soft && hipHostMalloc((void **)&sl.h_soft, b * sg::kCube * sizeof(double),
+                                          hipHostMallocDefault) != hipSuccess)) {
+            sogrand_server_destroy(s);
+            return SOGRAND_ENOMEM;
+        }
         if (hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess) {
             sogrand_server_destroy(s);
@@ -295,7 +369,17 @@ int sogrand_server_submit(sogrand_server *s, uint64_t id, const double *llr, siz
     if (!s || !llr || !info_bits || n_cw == 0) return SOGRAND_EINVAL;
     if (n_cw > s->batch_cw) return SOGRAND_EINVAL; // larger than one batch slot
     if (soft_out && !s->cfg.with_soft) return SOGRAND_EINVAL;
-    Request r{id, llr, sg::is_device_ptr(llr), n_cw, info_bits, soft_out, stats, 0.0};
+    Request r{id,
+              llr,
+              n_cw,
+              info_bits,
+              soft_out,
+              stats,
+              0.0,
+              sg::is_device_ptr(llr),
+              sg::is_device_ptr(info_bits),
+              soft_out && sg::is_device_ptr(soft_out),
+              stats && sg::is_device_ptr(stats)};
     {
         std::lock_guard<std::mutex> g(s->mu);
         if (s->fatal) return SOGRAND_EHIP;
@@ -351,6 +435,9 @@ void sogrand_server_destroy(sogrand_server *s)
     for (Slot &sl : s->slots) {
         if (sl.st) (void)hipStreamSynchronize(sl.st);
         if (sl.base) (void)hipFree(sl.base);
+        if (sl.h_out) (void)hipHostFree(sl.h_out);
+        if (sl.h_soft) (void)hipHostFree(sl.h_soft);
+        if (sl.h_llr) (void)hipHostFree(sl.h_llr);
         if (sl.done) (void)hipEventDestroy(sl.done);
         if (sl.st) (void)hipStreamDestroy(sl.st);
     }
