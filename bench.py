@@ -269,6 +269,12 @@ def workload_uneven(args) -> dict:
     d_bits = hip.DeviceArray((n, 64), np.uint8)
     d_st = hip.DeviceArray((n,), sg.CW_STATS_DTYPE)
     info = sg.gen_info_bits(args.seed, n)
+    # launch shape of the persistent kernel: autotuned on a sample of this workload
+    from sogrand_amd.autotune import autotune
+
+    n_tune = min(n, 32768)
+    tuned = autotune(dec, _View(d_llr, 0, (n_tune, 4096)), n_tune, log=log)
+    log(f"autotuned shape {tuned['best']}: {tuned['speedup_vs_default']}x the default")
     rows = []
     ref_bits = None
     for tmax, label in ((sg.UINT64_MAX, "unlimited"), (100000, "1e5"), (1000, "1e3")):
@@ -302,7 +308,9 @@ def workload_uneven(args) -> dict:
             "unit": "codewords/s", "n_gpus": 1, "dtype": "f64",
             "data": "synthetic (seeded info bits, (16,8)^3 encoding, BPSK, AWGN)",
             "config": {"workload": f"{n} codewords, per-codeword Eb/N0 ~ U[0.5, 3.5] dB, L=3, "
-                                   f"Imax=30", "n_cw": n},
+                                   f"Imax=30", "n_cw": n,
+                       "persistent_shape": tuned["best"]},
+            "autotune": {k: v for k, v in tuned.items() if k != "trials"},
             "rows": rows}
 
 

@@ -67,6 +67,9 @@ typedef struct sogrand_params {
                            (0 = tuned default)                                            */
     int32_t step_len;   /* persistent: guesses per lane between scheduling points
                            (0 = tuned default)                                            */
+    int32_t use_graph;  /* 1: when every buffer of a decode is device memory, capture its
+                           launch sequence in a hipGraph once per (buffers, size,
+                           parameters) and replay it on later calls                      */
 } sogrand_params;
 
 /* Per-codeword statistics (the reference prints only their averages,
@@ -110,6 +113,10 @@ int sogrand_bitso_lines(sogrand_ctx *ctx, const double *llr, size_t n_lines, dou
 /* Time (ms, HIP events on the launch stream) of the decode kernels of the last
  * sogrand_decode_cubic call on this context, and how many launches it made. */
 int sogrand_last_kernel_ms(sogrand_ctx *ctx, double *ms, uint32_t *launches);
+
+/* With use_graph: how many times the context's current captured graph has been
+ * replayed since it was captured (0 right after a capture). */
+int sogrand_graph_replays(sogrand_ctx *ctx, uint32_t *replays);
 
 /* The error-pattern schedule the decoder streams for intercept ic (ic >= 55
  * all share one order): 65535 16-bit masks over sorted positions (bit b = b-th

@@ -44,6 +44,7 @@ CW_STATS_DTYPE = np.dtype(
 EXPORTED = [
     "sogrand_params_default", "sogrand_strerror", "sogrand_ctx_create", "sogrand_ctx_set_params",
     "sogrand_ctx_destroy", "sogrand_decode_cubic", "sogrand_bitso_lines", "sogrand_last_kernel_ms",
+    "sogrand_graph_replays",
     "sogrand_sigma_for_ebn0", "sogrand_gen_info_bits", "sogrand_encode_cubes", "sogrand_gen_llr",
     "sogrand_gen_llr_device", "sogrand_schedule_table", "sogrand_server_cfg_default",
     "sogrand_server_create", "sogrand_server_batch_cw", "sogrand_server_submit",
@@ -72,6 +73,7 @@ class Params(C.Structure):
         ("waves_per_cu", C.c_int32),
         ("lane_batch", C.c_int32),
         ("step_len", C.c_int32),
+        ("use_graph", C.c_int32),
     ]
 
 
@@ -122,6 +124,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.sogrand_decode_cubic.argtypes = [vp, vp, C.c_size_t, vp, vp, vp, vp]
     lib.sogrand_bitso_lines.argtypes = [vp, vp, C.c_size_t, vp, vp, vp, vp]
     lib.sogrand_last_kernel_ms.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_uint32)]
+    lib.sogrand_graph_replays.argtypes = [vp, C.POINTER(C.c_uint32)]
     lib.sogrand_sigma_for_ebn0.argtypes = [C.c_double]
     lib.sogrand_sigma_for_ebn0.restype = C.c_double
     lib.sogrand_gen_info_bits.argtypes = [C.c_uint64, C.c_uint64, C.c_size_t, vp]
@@ -260,6 +263,11 @@ class Decoder:
         k = C.c_uint32()
         _check(self._lib.sogrand_last_kernel_ms(self._h, C.byref(ms), C.byref(k)), "last_kernel_ms")
         return ms.value, k.value
+
+    def graph_replays(self) -> int:
+        k = C.c_uint32()
+        _check(self._lib.sogrand_graph_replays(self._h, C.byref(k)), "graph_replays")
+        return k.value
 
     def gen_llr_device(self, seed: int, cw_offset: int, n: int, sigma, out, stream=None) -> None:
         sig = np.ascontiguousarray(np.atleast_1d(sigma), dtype=np.float64) if not hasattr(

@@ -230,6 +230,7 @@ void sogrand_ctx_destroy(sogrand_ctx *ctx)
     if (ctx->d_sched) (void)hipFree(ctx->d_sched);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;
 }
@@ -237,7 +238,8 @@ void sogrand_ctx_destroy(sogrand_ctx *ctx)
 
 // Decode one chunk whose buffers are all on the device.
 static int decode_device_chunk(sogrand_ctx *ctx, const double *d_llr, size_t n, uint8_t *d_info,
-                               double *d_soft, uint32_t *d_stats, hipStream_t st, float *ms)
+                               double *d_soft, uint32_t *d_stats, hipStream_t st, float *ms,
+                               bool use_graph = false)
 {
     const sogrand_params &p = ctx->p;
     if (ctx->status.reserve(n) != hipSuccess) return SOGRAND_ENOMEM;
@@ -254,7 +256,50 @@ static int decode_device_chunk(sogrand_ctx *ctx, const double *d_llr, size_t n, 
     if (ctx->queue.reserve(4) != hipSuccess) return SOGRAND_ENOMEM;
     job.le_scratch = ctx->le_scratch.p;
     job.queue = ctx->queue.p;
-    if (sg::launch_cubes(p, lp, job, st, ctx->ev0) != hipSuccess) return SOGRAND_EHIP;
+    if (use_graph) {
+        // the captured sequence is the queue-reset fill kernel and the decode
+        // kernel (no memset nodes); recapture whenever buffers, size, stream or
+        // parameters change
+        GraphKey k;
+        std::memset(&k, 0, sizeof k); // compared bytewise, padding included
+        k.llr = d_llr;
+        k.info = d_info;
+        k.soft = d_soft;
+        k.stats = d_stats;
+        k.le = job.le_scratch;
+        k.queue = job.queue;
+        k.status = job.status;
+        k.n = n;
+        k.st = st;
+        std::memcpy(&k.p, &p, sizeof p);
+        if (!ctx->graph || std::memcmp(&k, &ctx->graph_key, sizeof k) != 0) {
+            if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
+            ctx->graph = nullptr;
+            hipGraph_t g = nullptr;
+            if (hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed) != hipSuccess)
+                return SOGRAND_EHIP;
+            const hipError_t e1 = sg::launch_cubes(p, lp, job, st, nullptr);
+            const hipError_t e2 = hipStreamEndCapture(st, &g);
+            if (e1 != hipSuccess || e2 != hipSuccess || !g) {
+                if (g) (void)hipGraphDestroy(g);
+                return SOGRAND_EHIP;
+            }
+            const hipError_t e3 = hipGraphInstantiate(&ctx->graph, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e3 != hipSuccess) {
+                ctx->graph = nullptr;
+                return SOGRAND_EHIP;
+            }
+            std::memcpy(&ctx->graph_key, &k, sizeof k);
+            ctx->graph_replays = 0;
+        } else {
+            ctx->graph_replays += 1;
+        }
+        (void)hipEventRecord(ctx->ev0, st);
+        if (hipGraphLaunch(ctx->graph, st) != hipSuccess) return SOGRAND_EHIP;
+    } else if (sg::launch_cubes(p, lp, job, st, ctx->ev0) != hipSuccess) {
+        return SOGRAND_EHIP;
+    }
     (void)hipEventRecord(ctx->ev1, st);
     if (hipEventSynchronize(ctx->ev1) != hipSuccess) return SOGRAND_EHIP;
     float t = 0.f;
@@ -288,7 +333,8 @@ int sogrand_decode_cubic(sogrand_ctx *ctx, const double *llr, size_t n_cw, uint8
     float ms = 0.f;
     int rc = SOGRAND_OK;
     if (all_dev) {
-        rc = decode_device_chunk(ctx, llr, n_cw, info_bits, soft_out, (uint32_t *)stats, st, &ms);
+        rc = decode_device_chunk(ctx, llr, n_cw, info_bits, soft_out, (uint32_t *)stats, st, &ms,
+                                 ctx->p.use_graph != 0);
         ctx->last_launches = 1;
     } else {
         const size_t chunk = 65536;
@@ -380,6 +426,14 @@ int sogrand_bitso_lines(sogrand_ctx *ctx, const double *llr, size_t n_lines, dou
         return SOGRAND_EHIP;
     for (size_t i = 0; i < n_lines; ++i)
         if (status[i]) return SOGRAND_ENONFINITE;
+    return SOGRAND_OK;
+}
+
+int sogrand_graph_replays(sogrand_ctx *ctx, uint32_t *replays)
+{
+    if (!ctx || !replays) return SOGRAND_EINVAL;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    *replays = ctx->graph ? ctx->graph_replays : 0u;
     return SOGRAND_OK;
 }
 

@@ -46,6 +46,14 @@ inline uint64_t effective_tmax(uint64_t t) { return t == 0 ? 0x7fffffffull : t; 
 
 } // namespace sg
 
+// A captured decode launch sequence and what it was captured for.
+struct GraphKey {
+    const void *llr, *info, *soft, *stats, *le, *queue, *status;
+    size_t n;
+    hipStream_t st;
+    sogrand_params p;
+};
+
 struct sogrand_ctx {
     int device = 0;
     sogrand_params p{};
@@ -60,6 +68,9 @@ struct sogrand_ctx {
     double last_ms = 0.0;
     uint32_t last_launches = 0;
     int cu_count = 256;
+    hipGraphExec_t graph = nullptr;
+    GraphKey graph_key{};
+    uint32_t graph_replays = 0; // replays of the cached graph (tests)
     std::mutex mu;
 };
 

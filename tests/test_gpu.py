@@ -235,3 +235,35 @@ def test_cli_drop_in(tmp_path, oracle):
     assert lines[3] == f"Average NG per info bit: {st['NG'].sum() / (12 * 512):.2f}"
     assert lines[4] == f"Average NG_p per block: {st['NG_p'].sum() / 12:.2f}"
     assert np.fromfile(fst, sg.CW_STATS_DTYPE).tolist() == st.tolist()
+
+
+@pytest.mark.parametrize("backend", [0, 1], ids=["plain", "persistent"])
+def test_graph_replays_match_eager(decoder, backend):
+    # a captured launch sequence replayed several times, with eager decodes on
+    # another context and host synchronisations in between; every replay must
+    # equal the eager result
+    sg = _sg()
+    from sogrand_amd import hip
+
+    n = 300
+    llr = sg.gen_llr(91, n, ebn0_db=np.linspace(1.0, 3.0, n))
+    other = sg.gen_llr(92, 150, ebn0_db=2.0)
+    decoder.set_params(backend=backend, use_graph=0)
+    ref = decoder.decode(llr, soft=True)
+    d_llr = hip.DeviceArray.from_host(llr)
+    d_bits = hip.DeviceArray((n, 64), np.uint8)
+    d_st = hip.DeviceArray((n,), sg.CW_STATS_DTYPE)
+    d_soft = hip.DeviceArray((n, 4096))
+    with sg.Decoder(0, backend=backend) as eager:
+        decoder.set_params(use_graph=1)
+        for rep in range(4):
+            d_bits.zero()
+            d_st.zero()
+            decoder.decode(d_llr, out_bits=d_bits, out_stats=d_st, out_soft=d_soft)
+            assert decoder.graph_replays() == rep
+            assert (d_bits.to_host() == ref.bits).all(), rep
+            assert (d_st.to_host() == ref.stats).all(), rep
+            assert (d_soft.to_host() == ref.soft).all(), rep
+            eager.decode(other)
+            hip.synchronize()
+    decoder.set_params(use_graph=0)
