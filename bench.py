@@ -378,6 +378,27 @@ def workload_mixed(args) -> dict:
 
 
 # ---------------------------------------------------------------------------
+# HBM counter calibration
+
+
+def workload_copy(args) -> dict:
+    """Three device-to-device copies of 1 GiB: a kernel of known traffic (1 GiB
+    read + 1 GiB written each) that scales the FETCH_SIZE / WRITE_SIZE counters."""
+    import sogrand_amd as sg  # noqa: F401  (loads the HIP runtime)
+    from sogrand_amd import hip
+
+    nbytes = 1 << 30
+    a = hip.DeviceArray((nbytes,), np.uint8)
+    b = hip.DeviceArray((nbytes,), np.uint8)
+    a.zero()
+    for _ in range(3):
+        hip.copy(b, a, nbytes)
+    hip.synchronize()
+    return {"metric": "copy calibration", "bytes_read_per_copy": nbytes,
+            "bytes_written_per_copy": nbytes, "copies": 3}
+
+
+# ---------------------------------------------------------------------------
 # headline
 
 
@@ -486,7 +507,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="headline",
-                    choices=["headline", "uneven", "mixed-precision", "serving"])
+                    choices=["headline", "uneven", "mixed-precision", "serving", "copy"])
     ap.add_argument("--n-cw", type=int, default=None)
     ap.add_argument("--ebn0", type=float, default=2.0)
     ap.add_argument("--seed", type=int, default=20260)
@@ -508,7 +529,8 @@ def main() -> int:
     if args.n_cw is None:
         args.n_cw = 100_000
     fn = {"headline": workload_headline, "uneven": workload_uneven,
-          "mixed-precision": workload_mixed, "serving": workload_serving}[args.workload]
+          "mixed-precision": workload_mixed, "serving": workload_serving,
+          "copy": workload_copy}[args.workload]
     print(json.dumps(fn(args)), flush=True)
     return 0
 

@@ -3,9 +3,8 @@
 // One lane decodes one (16,8) line. Per-line state lives in registers; the
 // per-position path-metric increments and H columns, which the pattern loop
 // reads at data-dependent sorted positions, live in LDS in a position-major,
-// slot-minor layout (record (b, slot) at index b * nslot + slot), so that lanes
-// reading the same position hit consecutive addresses and different positions
-// never collide on a bank.
+// slot-minor layout (record (b, slot) at index (b + 1) * NS + slot, row 0 a null
+// record), so that lanes reading the same position hit consecutive addresses.
 //
 // Numerics follow SOGRAND_C/cubic_decoder_final.c exactly (citations are to
 // that file): every double is formed by the same operations in the same order;
@@ -45,7 +44,7 @@ __device__ __forceinline__ void load_exp_table(ExpPair *lds, int tid)
 __device__ __forceinline__ double exp_neg(double pm, const ExpPair *tab)
 {
     const bool under = !(pm < 745.5);
-    const double x = under ? -745.5 : -pm;
+    const double x = -fmin(pm, 745.5);
     const double kd = __builtin_rint(x * kExpInvLn2N);
     double r = __builtin_fma(-kd, kExpLn2NHi, x);
     r = __builtin_fma(-kd, kExpLn2NLo, r);
@@ -86,17 +85,32 @@ __device__ __forceinline__ double exp_neg_f32(float pm)
     return (double)__builtin_amdgcn_exp2f(pm * -1.44269504088896341f);
 }
 
-// Per-lane LDS record arrays: delta (Rec) and (H column | natural position << 8),
-// 17 records per slot: sorted positions 0..15 and a null record 16.
-template <class Rec> struct LineLdsT {
-    Rec *delta;
-    uint16_t *hp;
-    int nslot;
-    int slot;
-    __device__ Rec &d(int b) const { return delta[b * nslot + slot]; }
-    __device__ uint16_t &h(int b) const { return hp[b * nslot + slot]; }
+// Per-lane LDS record columns of an NS-slot array: delta (Rec) and (H column |
+// natural position << 8) of sorted positions b = 0..15 at rows 1..16, and at
+// row 0 a null record (delta +0.0, H column 0) that the pattern loop reads as
+// position -1, the index v_ffbl gives an exhausted mask. The column pointers
+// fold the slot in, so a record address is one shift-add of b.
+template <class RecT, int NS> struct LineLdsT {
+    using Rec = RecT;
+    Rec *dcol;       // this slot's record of sorted position 0
+    uint16_t *hcol;
+    __device__ LineLdsT(Rec *delta, uint16_t *hp, int slot)
+        : dcol(delta + NS + slot), hcol(hp + NS + slot)
+    {
+    }
+    __device__ Rec &d(int b) const { return dcol[b * NS]; }
+    __device__ uint16_t &h(int b) const { return hcol[b * NS]; }
 };
-using LineLds = LineLdsT<double>;
+
+// Index of the lowest set bit, -1 for 0 (v_ffbl_b32). Written as the
+// instruction itself: as a C expression the compiler re-creates the zero case
+// with a select once the index feeds an address computation.
+__device__ __forceinline__ int lowest_bit(uint32_t x)
+{
+    int b;
+    asm("v_ffbl_b32 %0, %1" : "=v"(b) : "v"(x));
+    return b;
+}
 
 // JacLog (:762).
 __device__ __forceinline__ double jaclog(double x)
@@ -119,7 +133,7 @@ __device__ __forceinline__ int32_t cvtt_i32(double r)
 // kept as bit masks (starts / pending) instead of a recursion stack:
 // partitions of disjoint segments commute, so the processing order does not
 // change the result.
-__device__ __noinline__ void quicksort_lds(const LineLds S)
+template <class LDS> __device__ __noinline__ void quicksort_lds(const LDS S)
 {
     uint32_t starts = 1u | (1u << 16), pending = 1u;
     while (pending) {
@@ -175,7 +189,7 @@ __device__ __forceinline__ void list_put(LineList &ls, double e, uint32_t cw)
 // order: the reference's quicksort decides the order among equal magnitudes
 // (replayed by quicksort_lds), then PM_HD in natural order and the sorted
 // records. Returns PM_HD.
-__device__ __forceinline__ double setup_with_ties(const LineLds S)
+template <class LDS> __device__ __forceinline__ double setup_with_ties(const LDS S)
 {
 #pragma unroll 1
     for (int i = 0; i < 16; ++i) S.h(i) = (uint16_t)(hcol_of((unsigned)i) | (i << 8));
@@ -198,9 +212,9 @@ __device__ __forceinline__ double setup_with_ties(const LineLds S)
 // Move the natural-order records d(i) to their sorted positions rank_i (4 bits
 // per position in `rank`), one permutation cycle at a time, and write the H
 // column | natural position records alongside.
-template <class Rec>
-__device__ __forceinline__ void place_by_rank(const LineLdsT<Rec> &S, uint64_t rank)
+template <class LDS> __device__ __forceinline__ void place_by_rank(const LDS &S, uint64_t rank)
 {
+    using Rec = typename LDS::Rec;
     uint32_t placed = 0;
 #pragma unroll 1
     for (int start = 0; start < 16; ++start) {
@@ -240,8 +254,8 @@ __device__ __forceinline__ void place_by_rank(const LineLdsT<Rec> &S, uint64_t r
 //   4. with ties, the reference's quicksort decides the order among equal
 //      magnitudes, so it is replayed on the records (quicksort_lds).
 // Returns false if an input is not finite.
-template <class Src>
-__device__ __forceinline__ bool line_setup(const Src &src, const LineLds &S, uint32_t &hd,
+template <class Src, class LDS>
+__device__ __forceinline__ bool line_setup(const Src &src, const LDS &S, uint32_t &hd,
                                            uint32_t &s0, double &pm_hd, int32_t &ic)
 {
     bool finite = true;
@@ -296,8 +310,8 @@ __device__ __forceinline__ bool line_setup(const Src &src, const LineLds &S, uin
     } else {
         pm_hd = setup_with_ties(S);
     }
-    S.d(16) = 0.0; // null record read by exhausted masks in search_run
-    S.h(16) = 0;
+    S.d(-1) = 0.0; // null record read by exhausted masks in search_run
+    S.h(-1) = 0;
     return true;
 }
 
@@ -305,8 +319,8 @@ __device__ __forceinline__ bool line_setup(const Src &src, const LineLds &S, uin
 // floats ordered by position), IC from the two float order statistics, PM_HD
 // in double in natural order, increments computed in double and rounded to
 // float. Returns false if an input is not finite.
-template <class Src>
-__device__ __forceinline__ bool line_setup_f32(const Src &src, const LineLdsT<float> &S,
+template <class Src, class LDS>
+__device__ __forceinline__ bool line_setup_f32(const Src &src, const LDS &S,
                                                uint32_t &hd, uint32_t &s0, double &pm_hd,
                                                int32_t &ic)
 {
@@ -348,14 +362,14 @@ __device__ __forceinline__ bool line_setup_f32(const Src &src, const LineLdsT<fl
         S.d(i) = (float)(jaclog(ai) - jn);
     }
     place_by_rank(S, rank);
-    S.d(16) = 0.0f;
-    S.h(16) = 0;
+    S.d(-1) = 0.0f;
+    S.h(-1) = 0;
     return true;
 }
 
 // Set-up in the arithmetic of P.
-template <class P, class Src>
-__device__ __forceinline__ bool line_setup_p(const Src &src, const LineLdsT<typename P::Rec> &S,
+template <class P, class Src, class LDS>
+__device__ __forceinline__ bool line_setup_p(const Src &src, const LDS &S,
                                              uint32_t &hd, uint32_t &s0, double &pm_hd,
                                              int32_t &ic)
 {
@@ -423,13 +437,13 @@ __device__ __forceinline__ void search_init(SearchState &ss, LineList &ls, uint3
 // chains; the guesses are still accounted strictly in schedule order (the
 // second is dropped if the first ends the search). Masks stream 8 per 16-byte
 // load with the next load issued one chunk ahead.
-template <bool FINITE_T, class P = PrecF64>
-__device__ __forceinline__ void search_run(SearchState &ss, LineList &ls,
-                                           const LineLdsT<typename P::Rec> &S, uint32_t hd,
+template <bool FINITE_T, class P, class LDS>
+__device__ __forceinline__ void search_run(SearchState &ss, LineList &ls, const LDS &S, uint32_t hd,
                                            uint32_t s0, double pm_hd, int32_t ic,
                                            const DecodeCfg &cfg, uint32_t max_steps)
 {
     using Rec = typename P::Rec;
+    static_assert(sizeof(Rec) == sizeof(typename LDS::Rec), "record type");
     const Rec pm_base = (Rec)pm_hd;
 #pragma unroll 1
     for (uint32_t step = 0; step < max_steps && !ss.done; step += 2) {
@@ -467,15 +481,15 @@ __device__ __forceinline__ void search_run(SearchState &ss, LineList &ls,
         }
         // path metrics (ascending sorted position, as getPM adds them) and
         // syndromes, two set bits of each pattern per iteration; an exhausted
-        // mask reads the null record 16 (delta +0.0, H column 0), which leaves
-        // both sums unchanged, so no lane needs a select or a branch
+        // mask gives position -1, the null record (delta +0.0, H column 0),
+        // which leaves both sums unchanged, so no lane needs a select or a branch
         Rec pm1 = pm_base, pm2 = pm_base;
         uint32_t syn1 = s0, syn2 = s0;
         uint32_t r1 = m1, r2 = take2 ? m2 : 0u;
         do {
             const uint32_t q1 = r1 & (r1 - 1u), q2 = r2 & (r2 - 1u);
-            const int b1 = __builtin_ctz(r1 | 0x10000u), c1 = __builtin_ctz(q1 | 0x10000u);
-            const int b2 = __builtin_ctz(r2 | 0x10000u), c2 = __builtin_ctz(q2 | 0x10000u);
+            const int b1 = lowest_bit(r1), c1 = lowest_bit(q1);
+            const int b2 = lowest_bit(r2), c2 = lowest_bit(q2);
             const Rec d1 = S.d(b1), e1 = S.d(c1), d2 = S.d(b2), e2 = S.d(c2);
             syn1 ^= (uint32_t)S.h(b1) ^ (uint32_t)S.h(c1);
             syn2 ^= (uint32_t)S.h(b2) ^ (uint32_t)S.h(c2);
@@ -519,8 +533,8 @@ __device__ __forceinline__ void search_run(SearchState &ss, LineList &ls,
     }
 }
 
-template <bool FINITE_T, class P = PrecF64>
-__device__ __forceinline__ void line_search(const LineLdsT<typename P::Rec> &S, uint32_t hd,
+template <bool FINITE_T, class P, class LDS>
+__device__ __forceinline__ void line_search(const LDS &S, uint32_t hd,
                                             uint32_t s0, double pm_hd, int32_t ic,
                                             const DecodeCfg &cfg, LineList &ls)
 {

@@ -90,7 +90,7 @@ __global__ __launch_bounds__(kPlainThreads) void plain_kernel(CubeJob job)
     const int t = threadIdx.x;
     load_exp_table(s_exp, t);
     __syncthreads();
-    const LineLdsT<Rec> S{s_delta, s_hp, kPlainThreads, t};
+    const LineLdsT<Rec, kPlainThreads> S(s_delta, s_hp, t);
     const DecodeCfg cfg{job.tmax, job.thres, job.sched, job.L, s_exp};
     double *le = job.le_scratch + (size_t)blockIdx.x * kCube;
 
@@ -204,7 +204,7 @@ __global__ __launch_bounds__(256) void lines_kernel(const double *llr, size_t n,
     cfg.exptab = s_exp;
     const size_t line = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (line >= n) return;
-    const LineLdsT<Rec> S{s_delta, s_hp, 256, (int)threadIdx.x};
+    const LineLdsT<Rec, 256> S(s_delta, s_hp, (int)threadIdx.x);
     const double *x = llr + line * 16;
     auto src = [x](int i) { return x[i]; };
     uint32_t hd, s0;

@@ -151,7 +151,7 @@ __global__ __launch_bounds__(64 * kWaves) void persistent_kernel(CubeJob job, ui
     const uint64_t lt_mask = (1ull << lane) - 1ull;
     Rec *rec_d = &s_d[wave][0];
     uint16_t *rec_h = &s_h[wave][0];
-    const LineLdsT<Rec> L{rec_d, rec_h, 64, lane};
+    const LineLdsT<Rec, 64> L(rec_d, rec_h, lane);
     const DecodeCfg cfg{job.tmax, job.thres, job.sched, job.L, s_exp};
     const size_t wave_id = (size_t)blockIdx.x * kWaves + wave;
     uint32_t *hd_bits = &s_hd[wave][0][0];
@@ -362,7 +362,7 @@ __global__ __launch_bounds__(64 * kWaves) void persistent_kernel(CubeJob job, ui
                 const int src = owner >= 0 ? owner : 0;
                 const int o_slot = __shfl(my_slot, src, 64);
                 const int o_line = __shfl(my_line, src, 64);
-                const LineLdsT<Rec> R{rec_d, rec_h, 64, src}; // the owner's records
+                const LineLdsT<Rec, 64> R(rec_d, rec_h, src); // the owner's records
                 bool fin = true, tie = false, neg = false;
                 double ai = 0.0, delta = 0.0;
                 Rec key = 0;
@@ -427,8 +427,8 @@ __global__ __launch_bounds__(64 * kWaves) void persistent_kernel(CubeJob job, ui
                     R.h((int)rank) = (uint16_t)(hcol_of((unsigned)i) | (i << 8));
                 }
                 if (myq >= 0) {
-                    L.d(16) = (Rec)0; // null record read by exhausted masks in search_run
-                    L.h(16) = 0;
+                    L.d(-1) = (Rec)0; // null record read by exhausted masks in search_run
+                    L.h(-1) = 0;
                     if (line_bad) {
                         atomicOr(&cnt[4 * my_slot + 3], 1u);
                         ls.nl = 0;

@@ -60,6 +60,11 @@ def synchronize() -> None:
     _ok(runtime().hipDeviceSynchronize(), "hipDeviceSynchronize")
 
 
+def copy(dst, src, nbytes: int) -> None:
+    """Device-to-device copy of nbytes (the runtime's copy kernel)."""
+    _ok(runtime().hipMemcpy(dst.data_ptr(), src.data_ptr(), nbytes, D2D), "hipMemcpy D2D")
+
+
 class DeviceArray:
     """A contiguous device buffer with a numpy shape/dtype."""
 
