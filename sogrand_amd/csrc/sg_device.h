@@ -165,25 +165,26 @@ template <class LDS> __device__ __noinline__ void quicksort_lds(const LDS S)
     }
 }
 
-// Result of the list search of one line.
+// Result of the list search of one line. The search keeps only what its
+// stopping rule needs: the running found mass of the list (getLConf's P_pos,
+// :769, summed in list order, so a running sum gives the same double), the
+// last P_neg and, per entry, the 16-bit position of its pattern in the
+// schedule (0 for the hard decision, else schedule index + 1). The entries'
+// weights and codewords, needed only by the soft output, are rebuilt from
+// those positions by list_entry with the same arithmetic.
 struct LineList {
-    uint32_t T;         // guesses (:625, :670, :711)
-    int nl;             // list entries
-    double e[kMaxL];    // exp(-PM_j) of each entry
-    uint32_t cw[kMaxL]; // entry codewords, natural order
-    double pnl;         // P_neg of the last insertion (getLConf side effect, :772)
+    uint32_t T;     // guesses (:625, :670, :711)
+    int nl;         // list entries
+    double ppos;    // sum of exp(-PM_j) over the entries, in list order
+    double pnl;     // P_neg of the last insertion (getLConf side effect, :772)
+    uint64_t code;  // 16 bits per entry: schedule position + 1, 0 = hard decision
 };
 
-__device__ __forceinline__ void list_put(LineList &ls, double e, uint32_t cw)
-{
-    // explicit selects keep the list in registers (no dynamic indexing)
-#pragma unroll
-    for (int q = 0; q < kMaxL; ++q)
-        if (ls.nl == q) {
-            ls.e[q] = e;
-            ls.cw[q] = cw;
-        }
-}
+// One list entry: exp(-PM) and the codeword (natural order).
+struct ListEntries {
+    double e[kMaxL];
+    uint32_t cw[kMaxL];
+};
 
 // Set-up of a line whose magnitudes contain a tie, from d(i) = |x_i| in natural
 // order: the reference's quicksort decides the order among equal magnitudes
@@ -380,19 +381,56 @@ __device__ __forceinline__ bool line_setup_p(const Src &src, const LDS &S,
 }
 
 // Record a list entry and apply the stopping rule (:628-641, :674-686, getLConf :767-774).
-__device__ __forceinline__ bool list_hit(LineList &ls, double e, uint32_t cw, double pnot,
+__device__ __forceinline__ bool list_hit(LineList &ls, double e, uint32_t code, double pnot,
                                          const DecodeCfg &cfg)
 {
-    list_put(ls, e, cw);
-    double ppos = 0.0;
-#pragma unroll
-    for (int q = 0; q < kMaxL; ++q)
-        if (q <= ls.nl) ppos += ls.e[q];
-    double pneg = 0.00390625 * pnot; // 2^-(n-k) * P_notGuess
+    ls.ppos += e;
+    const double pneg = 0.00390625 * pnot; // 2^-(n-k) * P_notGuess
     ls.pnl = pneg;
-    double conf = (ppos + pneg > 1e-40) ? ppos / (ppos + pneg) : 1.0;
+    const double conf = (ls.ppos + pneg > 1e-40) ? ls.ppos / (ls.ppos + pneg) : 1.0;
+    ls.code |= (uint64_t)code << (16 * ls.nl);
     ls.nl += 1;
     return conf > cfg.thres || (uint32_t)ls.nl == cfg.L;
+}
+
+// Entry q (< ls.nl) of a line's list, rebuilt from its schedule position with
+// the search's own arithmetic: the same ascending path-metric sum in the
+// precision of P and the same exp, so e and cw equal what the search found.
+template <class P, class LDS>
+__device__ __forceinline__ void list_entry(const LineList &ls, int q, const LDS &S, uint32_t hd,
+                                           double pm_hd, int32_t ic, const DecodeCfg &cfg,
+                                           double &e, uint32_t &cw)
+{
+    using Rec = typename P::Rec;
+    const uint32_t code = (uint32_t)(ls.code >> (16 * q)) & 0xffffu;
+    cw = hd;
+    if (code == 0) {
+        e = exp_neg(pm_hd, cfg.exptab);
+        return;
+    }
+    const int tbl = ic < kSchedGenericIc ? ic : kSchedGenericIc;
+    const uint32_t m = cfg.sched[(size_t)tbl * kSchedStride + (code - 1u)];
+    Rec pm = (Rec)pm_hd;
+#pragma unroll 1
+    for (uint32_t r = m; r; r &= r - 1u) {
+        const int b = __builtin_ctz(r);
+        pm = pm + S.d(b);
+        cw ^= 1u << (S.h(b) >> 8);
+    }
+    e = exp_neg((double)pm, cfg.exptab);
+}
+
+template <class P, class LDS>
+__device__ __forceinline__ void list_entries(const LineList &ls, const LDS &S, uint32_t hd,
+                                             double pm_hd, int32_t ic, const DecodeCfg &cfg,
+                                             ListEntries &out)
+{
+#pragma unroll
+    for (int q = 0; q < kMaxL; ++q) {
+        out.e[q] = 0.0;
+        out.cw[q] = 0;
+        if (q < ls.nl) list_entry<P>(ls, q, S, hd, pm_hd, ic, cfg, out.e[q], out.cw[q]);
+    }
 }
 
 // State of one line's guessing loop (sogrand_main_logic :624-744), so that
@@ -414,16 +452,13 @@ __device__ __forceinline__ void search_init(SearchState &ss, LineList &ls, uint3
 {
     ls.nl = 0;
     ls.pnl = 0.0;
-#pragma unroll
-    for (int q = 0; q < kMaxL; ++q) {
-        ls.e[q] = 0.0;
-        ls.cw[q] = 0;
-    }
+    ls.ppos = 0.0;
+    ls.code = 0;
     ss.T = 1;
     const double e_hd = exp_neg(pm_hd, cfg.exptab);
     ss.pnot = 1.0 - e_hd;
     ss.done = false;
-    if (s0 == 0) ss.done = list_hit(ls, e_hd, hd, ss.pnot, cfg);
+    if (s0 == 0) ss.done = list_hit(ls, e_hd, 0u, ss.pnot, cfg);
     const int tbl = ic < kSchedGenericIc ? ic : kSchedGenericIc;
     ss.tp = reinterpret_cast<const uint4 *>(cfg.sched + (size_t)tbl * kSchedStride);
     ss.nxt = ss.tp[0];
@@ -512,11 +547,7 @@ __device__ __forceinline__ void search_run(SearchState &ss, LineList &ls, const 
         ss.T += 1;
         ss.t = t + 1;
         ss.pnot -= e1;
-        if (hit1) {
-            uint32_t cw = hd;
-            for (uint32_t r = m1; r; r &= r - 1u) cw ^= 1u << (S.h(__builtin_ctz(r)) >> 8);
-            ss.done = list_hit(ls, e1, cw, ss.pnot, cfg);
-        }
+        if (hit1) ss.done = list_hit(ls, e1, t + 1u, ss.pnot, cfg);
         if (ss.done) break;
         if (!take2) { // end of the schedule, or the next weight level is over budget
             ss.done = true;
@@ -525,11 +556,7 @@ __device__ __forceinline__ void search_run(SearchState &ss, LineList &ls, const 
         ss.T += 1;
         ss.t = t + 2;
         ss.pnot -= e2;
-        if (hit2) {
-            uint32_t cw = hd;
-            for (uint32_t r = m2; r; r &= r - 1u) cw ^= 1u << (S.h(__builtin_ctz(r)) >> 8);
-            ss.done = list_hit(ls, e2, cw, ss.pnot, cfg);
-        }
+        if (hit2) ss.done = list_hit(ls, e2, t + 2u, ss.pnot, cfg);
     }
 }
 
@@ -545,7 +572,8 @@ __device__ __forceinline__ void line_search(const LDS &S, uint32_t hd,
 }
 
 // Soft output of position i (SOGRAND_bitSO :536-575) given the line input x.
-__device__ __forceinline__ double soft_app(const LineList &ls, double pnotin, int i, double x)
+__device__ __forceinline__ double soft_app(const ListEntries &ls, int nl, double pnotin, int i,
+                                           double x)
 {
     const double eps = 1e-40;
     double pp1 = 1.0 / (1.0 + exp(x));
@@ -555,7 +583,7 @@ __device__ __forceinline__ double soft_app(const LineList &ls, double pnotin, in
     double p1 = 0.0, p0 = 0.0;
 #pragma unroll
     for (int j = 0; j < kMaxL; ++j)
-        if (j < ls.nl) {
+        if (j < nl) {
             double c = (double)((ls.cw[j] >> i) & 1u);
             p1 += c * ls.e[j];
             p0 += (1.0 - c) * ls.e[j];

@@ -118,6 +118,8 @@ __global__ __launch_bounds__(kPlainThreads) void plain_kernel(CubeJob job)
                     ls.pnl = 0.0;
                 }
                 const double pnotin = fmax(ls.pnl, 1e-40);
+                ListEntries ent;
+                list_entries<P>(ls, S, hd, pm_hd, ic, cfg, ent);
 #pragma unroll 1
                 for (int i = 0; i < 16; ++i) {
                     const int p = src.base + i * src.stride;
@@ -127,7 +129,7 @@ __global__ __launch_bounds__(kPlainThreads) void plain_kernel(CubeJob job)
                         la = xin;
                         lenew = 0.0;
                     } else {
-                        la = soft_app(ls, pnotin, i, xin);
+                        la = soft_app(ent, ls.nl, pnotin, i, xin);
                         lenew = la - xin;
                     }
                     le[p] = lenew;
@@ -220,6 +222,8 @@ __global__ __launch_bounds__(256) void lines_kernel(const double *llr, size_t n,
         ls.pnl = 0.0;
     }
     const double pnotin = fmax(ls.pnl, 1e-40);
+    ListEntries ent;
+    list_entries<P>(ls, S, hd, pm_hd, ic, cfg, ent);
 #pragma unroll 1
     for (int i = 0; i < 16; ++i) {
         const double xi = x[i];
@@ -228,7 +232,7 @@ __global__ __launch_bounds__(256) void lines_kernel(const double *llr, size_t n,
             a = xi;
             e = 0.0;
         } else {
-            a = soft_app(ls, pnotin, i, xi);
+            a = soft_app(ent, ls.nl, pnotin, i, xi);
             e = a - xi;
         }
         if (la) la[line * 16 + i] = a;

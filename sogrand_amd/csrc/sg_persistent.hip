@@ -301,10 +301,27 @@ __global__ __launch_bounds__(64 * kWaves) void persistent_kernel(CubeJob job, ui
                 LineList ol;
                 ol.nl = __shfl(ls.nl, src, 64);
                 ol.pnl = shfl_d(ls.pnl, src);
+                const uint32_t code_lo = (uint32_t)__shfl((int)(uint32_t)ls.code, src, 64);
+                const uint32_t code_hi = (uint32_t)__shfl((int)(uint32_t)(ls.code >> 32), src, 64);
+                ol.code = ((uint64_t)code_hi << 32) | code_lo;
+                // lanes 16q + j (j < list size) rebuild entry j of the owner's
+                // list from its records, then the entries go to all 16 lanes
+                double my_e = 0.0;
+                uint32_t my_cw = 0;
+                {
+                    const uint32_t o_hd = (uint32_t)__shfl((int)hd, src, 64);
+                    const int32_t o_ic = __shfl(ic, src, 64);
+                    const double o_pm_hd = shfl_d(pm_hd, src);
+                    if (owner >= 0 && i < ol.nl) {
+                        const LineLdsT<Rec, 64> R(rec_d, rec_h, src);
+                        list_entry<P>(ol, i, R, o_hd, o_pm_hd, o_ic, cfg, my_e, my_cw);
+                    }
+                }
+                ListEntries ent;
 #pragma unroll
                 for (int j = 0; j < kMaxL; ++j) {
-                    ol.e[j] = shfl_d(ls.e[j], src);
-                    ol.cw[j] = (uint32_t)__shfl((int)ls.cw[j], src, 64);
+                    ent.e[j] = shfl_d(my_e, (lane & ~15) + j);
+                    ent.cw[j] = (uint32_t)__shfl((int)my_cw, (lane & ~15) + j, 64);
                 }
                 if (owner >= 0) {
                     int base, stride;
@@ -321,7 +338,7 @@ __global__ __launch_bounds__(64 * kWaves) void persistent_kernel(CubeJob job, ui
                         la = xin;
                         lenew = 0.0;
                     } else {
-                        la = soft_app(ol, fmax(ol.pnl, 1e-40), i, xin);
+                        la = soft_app(ent, ol.nl, fmax(ol.pnl, 1e-40), i, xin);
                         lenew = la - xin;
                     }
                     le[p] = lenew;
@@ -433,6 +450,8 @@ __global__ __launch_bounds__(64 * kWaves) void persistent_kernel(CubeJob job, ui
                         atomicOr(&cnt[4 * my_slot + 3], 1u);
                         ls.nl = 0;
                         ls.pnl = 0.0;
+                        ls.ppos = 0.0;
+                        ls.code = 0;
                         ss.T = 0;
                         ss.done = true;
                         state = kDone;
