@@ -21,18 +21,29 @@
 
 namespace sg {
 
-struct DecodeCfg {
-    uint64_t tmax;         // 0 already mapped to 0x7fffffff by the host
-    double thres;          // 1 - p_ET
-    const uint16_t *sched; // kSchedTables x kSchedStride masks
-    uint32_t L;            // list size 1..kMaxL
-    const ExpPair *exptab; // kExp2Table copied to LDS by the kernel
+// One entry of the LDS exp table: 2^(j/32) = hi + lo and the Taylor
+// coefficients 1/6! .. 1/3! of exp(r) - 1. The coefficients are the same in
+// every entry; reading them with the entry keeps them out of loop-invariant
+// registers, which the compiler otherwise pays for with a copy per step.
+struct ExpEntry {
+    double hi, lo;
+    double c6, c5, c4, c3;
 };
 
-// Copy the exp table into LDS (first 32 threads) - the caller synchronises.
-__device__ __forceinline__ void load_exp_table(ExpPair *lds, int tid)
+struct DecodeCfg {
+    uint64_t tmax;          // 0 already mapped to 0x7fffffff by the host
+    double thres;           // 1 - p_ET
+    const uint16_t *sched;  // kSchedTables x kSchedStride masks
+    uint32_t L;             // list size 1..kMaxL
+    const ExpEntry *exptab; // exp table in LDS (load_exp_table)
+};
+
+// Fill the LDS exp table (first 32 threads) - the caller synchronises.
+__device__ __forceinline__ void load_exp_table(ExpEntry *lds, int tid)
 {
-    if (tid < 32) lds[tid] = kExp2Table[tid];
+    if (tid < 32)
+        lds[tid] = ExpEntry{kExp2Table[tid].hi, kExp2Table[tid].lo, 1.0 / 720.0, 1.0 / 120.0,
+                            1.0 / 24.0, 1.0 / 6.0};
 }
 
 // exp(-pm) for pm >= 0, the probability weight of one guess (:626, :672, :712).
@@ -41,22 +52,22 @@ __device__ __forceinline__ void load_exp_table(ExpPair *lds, int tid)
 // result hi + (hi q + lo (1 + q)) scaled by 2^k. Within ~0.5 ulp (it differs
 // from the correctly rounded value on about 1 input in 200). Underflows to 0
 // beyond pm = 745.5, as exp does.
-__device__ __forceinline__ double exp_neg(double pm, const ExpPair *tab)
+__device__ __forceinline__ double exp_neg(double pm, const ExpEntry *tab)
 {
     const bool under = !(pm < 745.5);
     const double x = -fmin(pm, 745.5);
     const double kd = __builtin_rint(x * kExpInvLn2N);
+    const int k = (int)kd;
+    const ExpEntry &t = tab[k & 31];
     double r = __builtin_fma(-kd, kExpLn2NHi, x);
     r = __builtin_fma(-kd, kExpLn2NLo, r);
-    double q = 1.0 / 720.0;
-    q = __builtin_fma(q, r, 1.0 / 120.0);
-    q = __builtin_fma(q, r, 1.0 / 24.0);
-    q = __builtin_fma(q, r, 1.0 / 6.0);
+    double q = t.c6;
+    q = __builtin_fma(q, r, t.c5);
+    q = __builtin_fma(q, r, t.c4);
+    q = __builtin_fma(q, r, t.c3);
     q = __builtin_fma(q, r, 0.5);
     q = __builtin_fma(q, r, 1.0);
     q = q * r;
-    const int k = (int)kd;
-    const ExpPair t = tab[k & 31];
     const double s = __builtin_fma(t.lo, q, t.lo);
     const double m = __builtin_fma(t.hi, q, s) + t.hi;
     const double v = __builtin_ldexp(m, k >> 5);

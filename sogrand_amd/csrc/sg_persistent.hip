@@ -143,7 +143,7 @@ __global__ __launch_bounds__(64 * kWaves) void persistent_kernel(CubeJob job, ui
     __shared__ uint32_t s_cnt[kWaves][S][4]; // lines done, phase guess sum, phase max, bad
     __shared__ double s_stage_jn[kWaves][64];
     __shared__ double s_stage_a[kWaves][4][2]; // per sub-round line: |x| of ranks 0 and 7
-    __shared__ ExpPair s_exp[32];
+    __shared__ ExpEntry s_exp[32];
 
     load_exp_table(s_exp, threadIdx.x);
     __syncthreads(); // the only workgroup barrier: waves run independently from here
