@@ -452,8 +452,8 @@ struct SearchState {
     double pnot;      // P_notGuess (:614, :626, :672, :712)
     bool done;
     const uint4 *tp;  // schedule table of the line's intercept
-    uint4 nxt;        // prefetched chunk
-    uint64_t clo, chi;
+    uint4 nxt;        // next 16-byte chunk of 8 masks (prefetched)
+    uint4 cur;        // current chunk, rotated so that cur.x holds the next two masks
     int64_t cur_wt;   // weight level of the last pattern (finite Tmax only)
 };
 
@@ -473,7 +473,7 @@ __device__ __forceinline__ void search_init(SearchState &ss, LineList &ls, uint3
     const int tbl = ic < kSchedGenericIc ? ic : kSchedGenericIc;
     ss.tp = reinterpret_cast<const uint4 *>(cfg.sched + (size_t)tbl * kSchedStride);
     ss.nxt = ss.tp[0];
-    ss.clo = ss.chi = 0;
+    ss.cur = uint4{0u, 0u, 0u, 0u};
     ss.t = 0;
     ss.cur_wt = -1;
 }
@@ -482,7 +482,9 @@ __device__ __forceinline__ void search_init(SearchState &ss, LineList &ls, uint3
 // iteration so that each lane carries two independent path-metric and exp
 // chains; the guesses are still accounted strictly in schedule order (the
 // second is dropped if the first ends the search). Masks stream 8 per 16-byte
-// load with the next load issued one chunk ahead.
+// load with the next load issued one chunk ahead. The loop runs a step count
+// common to the wave and leaves when no lane is still searching; a finished
+// lane idles inside it, which keeps the wave's control flow to one mask.
 template <bool FINITE_T, class P, class LDS>
 __device__ __forceinline__ void search_run(SearchState &ss, LineList &ls, const LDS &S, uint32_t hd,
                                            uint32_t s0, double pm_hd, int32_t ic,
@@ -492,21 +494,21 @@ __device__ __forceinline__ void search_run(SearchState &ss, LineList &ls, const 
     static_assert(sizeof(Rec) == sizeof(typename LDS::Rec), "record type");
     const Rec pm_base = (Rec)pm_hd;
 #pragma unroll 1
-    for (uint32_t step = 0; step < max_steps && !ss.done; step += 2) {
+    for (uint32_t step = 0; step < max_steps; step += 2) {
+        if (__ballot(!ss.done) == 0ull) break;
+        if (ss.done) continue;
         const uint32_t t = ss.t; // always even here
         if ((t & 7u) == 0) {
-            ss.clo = (uint64_t)ss.nxt.x | ((uint64_t)ss.nxt.y << 32);
-            ss.chi = (uint64_t)ss.nxt.z | ((uint64_t)ss.nxt.w << 32);
+            ss.cur = ss.nxt;
             ss.nxt = ss.tp[(t >> 3) + 1]; // the table buffer carries one chunk of padding
         }
-        const uint32_t k = t & 7u;
-        const uint64_t wd = (k < 4u) ? ss.clo : ss.chi;
-        const uint32_t sh = (k & 3u) * 16u;
-        const uint32_t m1 = (uint32_t)(wd >> sh) & 0xffffu;
-        const uint32_t m2 = (uint32_t)(wd >> (sh + 16u)) & 0xffffu;
+        const uint32_t m1 = ss.cur.x & 0xffffu, m2 = ss.cur.x >> 16;
+        ss.cur.x = ss.cur.y;
+        ss.cur.y = ss.cur.z;
+        ss.cur.z = ss.cur.w;
         if (m1 == 0) { // all 65535 patterns tested (wt > wt_max)
             ss.done = true;
-            break;
+            continue;
         }
         bool take2 = m2 != 0;
         if (FINITE_T) { // loop guard once per weight level (:652)
@@ -518,7 +520,7 @@ __device__ __forceinline__ void search_run(SearchState &ss, LineList &ls, const 
             if (wt1 != ss.cur_wt) {
                 if (!((uint64_t)ss.T < cfg.tmax)) {
                     ss.done = true;
-                    break;
+                    continue;
                 }
                 ss.cur_wt = wt1;
             }
@@ -558,16 +560,16 @@ __device__ __forceinline__ void search_run(SearchState &ss, LineList &ls, const 
         ss.T += 1;
         ss.t = t + 1;
         ss.pnot -= e1;
-        if (hit1) ss.done = list_hit(ls, e1, t + 1u, ss.pnot, cfg);
-        if (ss.done) break;
-        if (!take2) { // end of the schedule, or the next weight level is over budget
-            ss.done = true;
-            break;
+        bool stop = hit1 && list_hit(ls, e1, t + 1u, ss.pnot, cfg);
+        // end of the schedule, or the next weight level is over budget
+        if (!stop && !take2) stop = true;
+        if (!stop) {
+            ss.T += 1;
+            ss.t = t + 2;
+            ss.pnot -= e2;
+            stop = hit2 && list_hit(ls, e2, t + 2u, ss.pnot, cfg);
         }
-        ss.T += 1;
-        ss.t = t + 2;
-        ss.pnot -= e2;
-        if (hit2) ss.done = list_hit(ls, e2, t + 2u, ss.pnot, cfg);
+        ss.done = stop;
     }
 }
 
