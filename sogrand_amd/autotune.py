@@ -23,7 +23,7 @@ SPACE = {
     "step_len": [16, 32, 64, 128, 256, 512],
     "waves_per_cu": [0, 8, 4],
 }
-DEFAULT = {"wave_slots": 1, "lane_batch": 4, "step_len": 64, "waves_per_cu": 0}
+DEFAULT = {"wave_slots": 2, "lane_batch": 4, "step_len": 64, "waves_per_cu": 0}
 
 
 def _time(dec, d_llr, d_bits, d_st, shape, repeats):

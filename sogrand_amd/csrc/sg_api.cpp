@@ -44,7 +44,7 @@ LaunchPlan plan_launch(const sogrand_ctx *ctx, const sogrand_params &p, size_t n
     LaunchPlan lp{};
     lp.persistent = p.backend == SOGRAND_BACKEND_PERSISTENT;
     if (lp.persistent) {
-        const int slots = p.wave_slots > 0 ? p.wave_slots : 1;
+        const int slots = p.wave_slots > 0 ? p.wave_slots : 2;
         const int wpb = kPersistentWaves;
         // a persistent grid must be resident all at once: never more workgroups
         // per CU than the occupancy the kernel's registers and LDS allow
