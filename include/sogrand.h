@@ -61,7 +61,7 @@ typedef struct sogrand_params {
     double alpha;       /* extrinsic scaling                                             */
     int32_t precision;  /* SOGRAND_FP64 or SOGRAND_FP32_MIXED                            */
     int32_t backend;    /* SOGRAND_BACKEND_*                                             */
-    int32_t wave_slots; /* persistent: cube slots per wave, 1 or 2 (0 = tuned default, 2) */
+    int32_t wave_slots; /* persistent: cube slots per wave, 1..4 (0 = tuned default, 2)   */
     int32_t waves_per_cu; /* persistent: resident waves per CU (0 = as many as fit)      */
     int32_t lane_batch; /* persistent: free lanes gathered before new lines are handed out
                            (0 = tuned default)                                            */

@@ -18,7 +18,7 @@ import time
 import numpy as np
 
 SPACE = {
-    "wave_slots": [1, 2],
+    "wave_slots": [1, 2, 3, 4],
     "lane_batch": [1, 2, 4, 8, 16, 32],
     "step_len": [16, 32, 64, 128, 256, 512],
     "waves_per_cu": [0, 8, 4],

@@ -68,7 +68,7 @@ LaunchPlan plan_launch(const sogrand_ctx *ctx, const sogrand_params &p, size_t n
     return lp;
 }
 
-size_t max_le_slots(size_t n) { return n + 2 * (size_t)kPersistentWaves * 2; }
+size_t max_le_slots(size_t n) { return n + 2 * (size_t)kPersistentWaves * 4; }
 
 void fill_job(CubeJob &job, const sogrand_ctx *ctx, const sogrand_params &p)
 {
@@ -142,7 +142,7 @@ static int check_params(const sogrand_params *p)
     if (p->precision != SOGRAND_FP64 && p->precision != SOGRAND_FP32_MIXED) return SOGRAND_EINVAL;
     if (p->backend != SOGRAND_BACKEND_PLAIN && p->backend != SOGRAND_BACKEND_PERSISTENT)
         return SOGRAND_EINVAL;
-    if (p->wave_slots < 0 || p->wave_slots > 2 || p->waves_per_cu < 0 || p->waves_per_cu > 32)
+    if (p->wave_slots < 0 || p->wave_slots > 4 || p->waves_per_cu < 0 || p->waves_per_cu > 32)
         return SOGRAND_EINVAL;
     if (p->lane_batch < 0 || p->lane_batch > 64 || p->step_len < 0 || p->step_len > 65536)
         return SOGRAND_EINVAL;

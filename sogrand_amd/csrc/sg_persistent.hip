@@ -94,19 +94,14 @@ __device__ __forceinline__ bool plane_ok(const Word &word, bool rows)
 // Lane states of the persistent kernel.
 enum : int { kIdle = 0, kSearch = 1, kDone = 2, kNew = 3 };
 
+// Wave-uniform state of a wave's cube slots. The arrays are only ever indexed
+// by constants (fully unrolled slot loops); what a lane needs for the slot of
+// its own line (cube, axis, iteration) is mirrored in LDS, where a per-lane
+// index is cheap.
 template <int S> struct SlotState {
     uint32_t cube[S], it[S], ax[S], nxt_line[S], half[S];
     uint64_t ng[S], ngp[S];
     bool active[S];
-    // per-lane selection of a slot's value (s is per lane; the arrays are uniform)
-    __device__ uint32_t sel(const uint32_t (&v)[S], int s) const
-    {
-        uint32_t r = v[0];
-#pragma unroll
-        for (int k = 1; k < S; ++k)
-            if (s == k) r = v[k];
-        return r;
-    }
 };
 
 // The lowest four set lanes of a wave mask (owners of one transition sub-round),
@@ -143,6 +138,7 @@ __global__ __launch_bounds__(64 * kWaves) void persistent_kernel(CubeJob job, ui
     __shared__ uint32_t s_cnt[kWaves][S][4]; // lines done, phase guess sum, phase max, bad
     __shared__ double s_stage_jn[kWaves][64];
     __shared__ double s_stage_a[kWaves][4][2]; // per sub-round line: |x| of ranks 0 and 7
+    __shared__ uint32_t s_geom[kWaves][S][2];  // per slot: cube, axis | iteration << 8
     __shared__ ExpEntry s_exp[32];
 
     load_exp_table(s_exp, threadIdx.x);
@@ -157,10 +153,11 @@ __global__ __launch_bounds__(64 * kWaves) void persistent_kernel(CubeJob job, ui
     uint32_t *hd_bits = &s_hd[wave][0][0];
     uint32_t *cnt = &s_cnt[wave][0][0];
     double *stage_jn = &s_stage_jn[wave][0];
+    uint32_t *geom = &s_geom[wave][0][0];
     double *stage_a = &s_stage_a[wave][0][0];
 
     SlotState<S> sl;
-#pragma unroll
+#pragma clang loop unroll(full)
     for (int s = 0; s < S; ++s) {
         sl.active[s] = false;
         sl.cube[s] = sl.it[s] = sl.ax[s] = sl.nxt_line[s] = sl.half[s] = 0;
@@ -183,7 +180,8 @@ __global__ __launch_bounds__(64 * kWaves) void persistent_kernel(CubeJob job, ui
     // geometry of (slot, line) as seen by any lane
     auto line_geom = [&](int s, int line, int &base, int &stride, const double *&lch, double *&le,
                          double *&soft, bool &first) {
-        const uint32_t ax = sl.sel(sl.ax, s), c = sl.sel(sl.cube, s), it = sl.sel(sl.it, s);
+        const uint32_t c = geom[2 * s], g = geom[2 * s + 1];
+        const uint32_t ax = g & 0xffu, it = g >> 8;
         axis_geom((int)ax, line, base, stride);
         lch = job.llr + (size_t)c * kCube;
         le = job.le_scratch + (wave_id * S + (size_t)s) * kCube;
@@ -193,7 +191,7 @@ __global__ __launch_bounds__(64 * kWaves) void persistent_kernel(CubeJob job, ui
 
     for (;;) {
         // ---- 1. phase completion / cube retirement (wave-uniform) ----
-#pragma unroll
+#pragma clang loop unroll(full)
         for (int s = 0; s < S; ++s) {
             if (!sl.active[s] || sl.nxt_line[s] < 256) continue;
             const uint32_t done_lines = __builtin_amdgcn_readfirstlane(cnt[4 * s + 0]);
@@ -244,6 +242,7 @@ __global__ __launch_bounds__(64 * kWaves) void persistent_kernel(CubeJob job, ui
                     sl.ax[s] += 1;
                 }
                 sl.half[s] += 1; // n_iter += 0.5 before each phase (:145, :184, :223)
+                if (lane == 0) geom[2 * s + 1] = sl.ax[s] | (sl.it[s] << 8);
                 sl.nxt_line[s] = 0;
                 hd_bits[128 * s + lane] = 0;
                 hd_bits[128 * s + 64 + lane] = 0;
@@ -251,7 +250,7 @@ __global__ __launch_bounds__(64 * kWaves) void persistent_kernel(CubeJob job, ui
             }
         }
         // ---- 2. refill empty slots from the device queue ----
-#pragma unroll
+#pragma clang loop unroll(full)
         for (int s = 0; s < S; ++s) {
             if (sl.active[s] || queue_empty) continue;
             uint32_t c = 0;
@@ -268,13 +267,17 @@ __global__ __launch_bounds__(64 * kWaves) void persistent_kernel(CubeJob job, ui
             sl.half[s] = 1;
             sl.nxt_line[s] = 0;
             sl.ng[s] = sl.ngp[s] = 0;
+            if (lane == 0) {
+                geom[2 * s] = c;
+                geom[2 * s + 1] = 0u | (1u << 8);
+            }
             hd_bits[128 * s + lane] = 0;
             hd_bits[128 * s + 64 + lane] = 0;
             if (lane < 4) cnt[4 * s + lane] = 0;
         }
         bool any_active = false;
         uint32_t avail = 0;
-#pragma unroll
+#pragma clang loop unroll(full)
         for (int s = 0; s < S; ++s) {
             any_active = any_active || sl.active[s];
             if (sl.active[s]) avail += 256u - sl.nxt_line[s];
@@ -354,7 +357,7 @@ __global__ __launch_bounds__(64 * kWaves) void persistent_kernel(CubeJob job, ui
             }
             // 3b. hand the next lines of the active phases to free lanes
             uint64_t idle = __ballot(state == kIdle);
-#pragma unroll
+#pragma clang loop unroll(full)
             for (int s = 0; s < S; ++s) {
                 if (!sl.active[s] || sl.nxt_line[s] >= 256 || idle == 0ull) continue;
                 const uint32_t left = 256u - sl.nxt_line[s];
@@ -487,6 +490,10 @@ int persistent_blocks_per_cu(int slots, bool finite, bool f32)
         if (finite) { SG_OCC_P(true, 1); } else { SG_OCC_P(false, 1); }
     } else if (slots == 2) {
         if (finite) { SG_OCC_P(true, 2); } else { SG_OCC_P(false, 2); }
+    } else if (slots == 3) {
+        if (finite) { SG_OCC_P(true, 3); } else { SG_OCC_P(false, 3); }
+    } else if (slots == 4) {
+        if (finite) { SG_OCC_P(true, 4); } else { SG_OCC_P(false, 4); }
     }
 #undef SG_OCC_P
 #undef SG_OCC
@@ -513,6 +520,10 @@ hipError_t launch_persistent(const CubeJob &job, int waves_per_block, int blocks
         if (finite) { SG_LAUNCH_P(true, 1); } else { SG_LAUNCH_P(false, 1); }
     } else if (slots == 2) {
         if (finite) { SG_LAUNCH_P(true, 2); } else { SG_LAUNCH_P(false, 2); }
+    } else if (slots == 3) {
+        if (finite) { SG_LAUNCH_P(true, 3); } else { SG_LAUNCH_P(false, 3); }
+    } else if (slots == 4) {
+        if (finite) { SG_LAUNCH_P(true, 4); } else { SG_LAUNCH_P(false, 4); }
     } else {
         return hipErrorInvalidValue;
     }
