@@ -550,15 +550,17 @@ __device__ __forceinline__ void search_run(SearchState &ss, LineList &ls, const 
         double e1, e2;
         if (P::kF32) {
             // single-precision weights for the found-mass chain; a hit's list
-            // weight is taken in double from its (single-precision) metric
-            e1 = hit1 ? exp_neg((double)pm1, cfg.exptab) : exp_neg_f32((float)pm1);
-            e2 = hit2 ? exp_neg((double)pm2, cfg.exptab) : exp_neg_f32((float)pm2);
+            // weight is taken in double from its (single-precision) metric,
+            // in the (rare) hit branch below
+            e1 = exp_neg_f32((float)pm1);
+            e2 = exp_neg_f32((float)pm2);
         } else {
             e1 = exp_neg((double)pm1, cfg.exptab);
             e2 = exp_neg((double)pm2, cfg.exptab);
         }
         ss.T += 1;
         ss.t = t + 1;
+        if (P::kF32 && hit1) e1 = exp_neg((double)pm1, cfg.exptab);
         ss.pnot -= e1;
         bool stop = hit1 && list_hit(ls, e1, t + 1u, ss.pnot, cfg);
         // end of the schedule, or the next weight level is over budget
@@ -566,6 +568,7 @@ __device__ __forceinline__ void search_run(SearchState &ss, LineList &ls, const 
         if (!stop) {
             ss.T += 1;
             ss.t = t + 2;
+            if (P::kF32 && hit2) e2 = exp_neg((double)pm2, cfg.exptab);
             ss.pnot -= e2;
             stop = hit2 && list_hit(ls, e2, t + 2u, ss.pnot, cfg);
         }

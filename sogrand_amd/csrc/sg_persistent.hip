@@ -127,8 +127,12 @@ __device__ __forceinline__ double shfl_d(double v, int src)
     return bits_to_double(((uint64_t)hi << 32) | lo);
 }
 
+// Four waves per SIMD asked for: the single-precision variant's records fit
+// four workgroups' LDS per CU and its registers fit 128 with a few bytes of
+// spill, which measures faster than three waves; the FP64 kernel's LDS allows
+// three workgroups per CU whatever the hint says.
 template <bool FINITE_T, int S, class P>
-__global__ __launch_bounds__(64 * kWaves) void persistent_kernel(CubeJob job, uint32_t thresh,
+__global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))) void persistent_kernel(CubeJob job, uint32_t thresh,
                                                                  uint32_t step_len)
 {
     using Rec = typename P::Rec;
