@@ -162,7 +162,8 @@ def wilson(k: int, n: int, z: float = 1.96) -> tuple[float, float]:
 
 
 def serve_load(dec, capacity: float, load: float, seconds: float, seed: int,
-               max_inflight: int = 16, max_batch_cw: int = 0, plain_below_cw: int = 0) -> dict:
+               max_inflight: int = 16, max_batch_cw: int = 0, plain_below_cw: int = 0,
+               hold_above_cw: int = 0) -> dict:
     """Open-loop Poisson arrivals of frames (U{1..256} codewords, per-frame Eb/N0
     ~ U[1, 3] dB) at `load` x capacity, through the serving loop. Latency runs
     from a frame's scheduled arrival to its results being back in host memory."""
@@ -186,7 +187,7 @@ def serve_load(dec, capacity: float, load: float, seconds: float, seed: int,
     views = [_View(d_llr, int(offs[k]) * 4096 * 8, (int(sizes[k]), 4096)) for k in range(nf)]
     done = []
     with sg.Server(dec, hbm_budget=8 << 30, max_inflight=max_inflight, max_batch_cw=max_batch_cw,
-                   plain_below_cw=plain_below_cw) as srv:
+                   plain_below_cw=plain_below_cw, hold_above_cw=hold_above_cw) as srv:
         t0 = srv.clock() + 0.01
         lag = 0.0
         for k in range(nf):
@@ -241,7 +242,8 @@ def workload_serving(args) -> dict:
     pts = []
     for load in (0.1, 0.3, 0.5, 0.7, 0.9, 1.0, 1.2):
         r = serve_load(dec, cap, load, args.serve_seconds, args.seed + int(load * 100),
-                       args.serve_slots, args.serve_batch, args.serve_plain_below)
+                       args.serve_slots, args.serve_batch, args.serve_plain_below,
+                       args.serve_hold_above)
         log(f"load {load:.1f}: {r}")
         pts.append(r)
     return {"metric": "serving: p50/p99 frame latency and sustained codewords/s vs offered load",
@@ -250,7 +252,8 @@ def workload_serving(args) -> dict:
             "config": {"workload": f"serving loop, {args.serve_slots} batch slots, 8 GiB budget",
                        "seconds_per_point": args.serve_seconds, "slots": args.serve_slots,
                        "max_batch_cw": args.serve_batch,
-                       "plain_below_cw": args.serve_plain_below},
+                       "plain_below_cw": args.serve_plain_below,
+                       "hold_above_cw": args.serve_hold_above},
             "capacity_cw_s": round(cap, 1), "points": pts}
 
 
@@ -489,7 +492,7 @@ def workload_headline(args) -> dict:
     if not args.no_serve:
         cap = serving_capacity(dec, args.seed + 1)
         r = serve_load(dec, cap, 0.7, args.serve_seconds, args.seed + 2, args.serve_slots,
-                       args.serve_batch, args.serve_plain_below)
+                       args.serve_batch, args.serve_plain_below, args.serve_hold_above)
         log(f"serving at 70 % of {cap:.0f} cw/s: {r}")
         out["latency"] = {"p50_ms": r["p50_ms"], "p99_ms": r["p99_ms"],
                           "workload": "serving loop, Poisson frames of U{1..256} codewords at "
@@ -516,6 +519,7 @@ def main() -> int:
     ap.add_argument("--serve-slots", type=int, default=16)
     ap.add_argument("--serve-batch", type=int, default=0)
     ap.add_argument("--serve-plain-below", type=int, default=0)
+    ap.add_argument("--serve-hold-above", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serve", action="store_true")

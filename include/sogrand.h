@@ -172,8 +172,12 @@ typedef struct sogrand_server_cfg {
     uint32_t max_pending_cw; /* waiting codewords before submit refuses (default 2^20)     */
     int32_t with_soft;       /* reserve room for soft outputs (default 0)                  */
     uint32_t plain_below_cw; /* batches with fewer codewords run the plain kernel, which
-                                finishes a single pass over the GPU sooner (0 = 3 per CU;
-                                1 = always the configured backend)                        */
+                                finishes a cube sooner when the device has room
+                                (0 = 3 per CU; 1 = always the configured backend)         */
+    uint32_t hold_above_cw;  /* no batch is launched while this many codewords are
+                                decoding, so that waiting requests leave together as one
+                                larger batch (0 = no limit). A batch completes with its
+                                slowest cube, so holding costs throughput on uneven work  */
 } sogrand_server_cfg;
 
 typedef struct sogrand_completion {

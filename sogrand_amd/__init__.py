@@ -85,6 +85,7 @@ class ServerCfg(C.Structure):
         ("max_pending_cw", C.c_uint32),
         ("with_soft", C.c_int32),
         ("plain_below_cw", C.c_uint32),
+        ("hold_above_cw", C.c_uint32),
     ]
 
 
@@ -288,7 +289,7 @@ class Server:
 
     def __init__(self, decoder: Decoder, hbm_budget: int | None = None, max_inflight: int = 16,
                  max_batch_cw: int = 0, max_pending_cw: int = 1 << 20, with_soft: bool = False,
-                 plain_below_cw: int = 0):
+                 plain_below_cw: int = 0, hold_above_cw: int = 0):
         lib = decoder._lib
         self._lib = lib
         cfg = ServerCfg()
@@ -300,6 +301,7 @@ class Server:
         cfg.max_pending_cw = max_pending_cw
         cfg.with_soft = 1 if with_soft else 0
         cfg.plain_below_cw = plain_below_cw
+        cfg.hold_above_cw = hold_above_cw
         h = C.c_void_p()
         _check(lib.sogrand_server_create(C.byref(h), decoder._h, C.byref(cfg)), "server_create")
         self._h = h

@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <cstdint>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -72,7 +73,9 @@ struct sogrand_server {
     sogrand_params p{};
     sogrand_server_cfg cfg{};
     size_t batch_cw = 0;
-    size_t plain_below = 0; // batches smaller than this run the plain kernel
+    size_t plain_below = 0; // in-flight codewords below which batches run the plain kernel
+    size_t hold_above = 0;  // no new batch while this many codewords are in flight
+    size_t inflight_cw = 0; // codewords of the batches on the device (worker thread only)
     std::vector<Slot> slots;
     std::deque<Request> pending;
     size_t pending_cw = 0;
@@ -151,7 +154,13 @@ void sogrand_server::launch(Slot &s)
     job.n_cw = n;
     sg::fill_job(job, ctx, p);
     sogrand_params q = p;
-    if (n < plain_below) q.backend = SOGRAND_BACKEND_PLAIN; // one pass over the GPU: less latency
+    // A batch the plain kernel decodes in one pass over the device runs it:
+    // four waves per cube finish a cube sooner, and with many batch slots in
+    // flight small plain batches keep every CU busy. Larger batches (requests
+    // that piled up while all slots were busy) run the configured kernel.
+    // Both decode every codeword identically.
+    if (n < plain_below) q.backend = SOGRAND_BACKEND_PLAIN;
+    inflight_cw += n;
     const sg::LaunchPlan lp = sg::plan_launch(ctx, q, n);
     ok = ok && sg::launch_cubes(q, lp, job, s.st, nullptr) == hipSuccess;
     // results: device-resident outputs directly, host outputs via pinned staging
@@ -216,6 +225,9 @@ void sogrand_server::finish(Slot &s, bool event_ok)
         out.push_back(sogrand_completion{r.id, st, (uint32_t)r.n, r.t_submit, s.t_start, t});
         off += r.n;
     }
+    size_t n_done = 0;
+    for (const Request &r : s.reqs) n_done += r.n;
+    inflight_cw -= n_done;
     s.reqs.clear();
     s.busy = false;
     std::lock_guard<std::mutex> g(mu);
@@ -238,6 +250,9 @@ void sogrand_server::run()
         }
         for (Slot &s : slots) {
             if (s.busy) continue;
+            // optional: while the device is busy enough, requests accumulate
+            // and leave together as one larger batch
+            if (inflight_cw >= hold_above) break;
             {
                 std::lock_guard<std::mutex> g(mu);
                 take(s);
@@ -275,6 +290,7 @@ void sogrand_server_cfg_default(sogrand_server_cfg *c)
     c->max_pending_cw = 1u << 20;
     c->with_soft = 0;
     c->plain_below_cw = 0;
+    c->hold_above_cw = 0;
 }
 
 int sogrand_server_create(sogrand_server **out, sogrand_ctx *ctx, const sogrand_server_cfg *cfg)
@@ -309,6 +325,7 @@ int sogrand_server_create(sogrand_server **out, sogrand_ctx *ctx, const sogrand_
     }
     s->batch_cw = b;
     s->plain_below = c.plain_below_cw ? c.plain_below_cw : (size_t)3 * ctx->cu_count;
+    s->hold_above = c.hold_above_cw ? c.hold_above_cw : SIZE_MAX;
     if (hipSetDevice(ctx->device) != hipSuccess) {
         delete s;
         return SOGRAND_EHIP;
