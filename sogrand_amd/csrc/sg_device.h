@@ -444,6 +444,16 @@ __device__ __forceinline__ void list_entries(const LineList &ls, const LDS &S, u
     }
 }
 
+// Sum of (b + 1) over the set bits b of a 16-bit mask: the landslide weight
+// of the pattern without its intercept part (wt = IC w + S, :646-653). Bit k of
+// the position b is counted by popcount(m & the mask of positions with bit k).
+__device__ __forceinline__ int position_sum(uint32_t m)
+{
+    return __builtin_popcount(m) + __builtin_popcount(m & 0xAAAAu) +
+           2 * __builtin_popcount(m & 0xCCCCu) + 4 * __builtin_popcount(m & 0xF0F0u) +
+           8 * __builtin_popcount(m & 0xFF00u);
+}
+
 // State of one line's guessing loop (sogrand_main_logic :624-744), so that
 // the loop can be run in bounded steps.
 struct SearchState {
@@ -512,11 +522,8 @@ __device__ __forceinline__ void search_run(SearchState &ss, LineList &ls, const 
         }
         bool take2 = m2 != 0;
         if (FINITE_T) { // loop guard once per weight level (:652)
-            int s1 = 0, s2 = 0;
-            for (uint32_t r = m1; r; r &= r - 1u) s1 += __builtin_ctz(r) + 1;
-            for (uint32_t r = m2; r; r &= r - 1u) s2 += __builtin_ctz(r) + 1;
-            const int64_t wt1 = (int64_t)ic * __builtin_popcount(m1) + s1;
-            const int64_t wt2 = (int64_t)ic * __builtin_popcount(m2) + s2;
+            const int64_t wt1 = (int64_t)ic * __builtin_popcount(m1) + position_sum(m1);
+            const int64_t wt2 = (int64_t)ic * __builtin_popcount(m2) + position_sum(m2);
             if (wt1 != ss.cur_wt) {
                 if (!((uint64_t)ss.T < cfg.tmax)) {
                     ss.done = true;
