@@ -401,6 +401,20 @@ def workload_copy(args) -> dict:
             "bytes_written_per_copy": nbytes, "copies": 3}
 
 
+def measured_traffic(n: int, ebn0: float):
+    """HBM bytes per launch of the persistent kernel on this workload, from the
+    committed counter runs (FETCH_SIZE and WRITE_SIZE scaled by a copy kernel
+    of known size); None for another workload."""
+    path = os.path.join(ROOT, "profiles", "persistent_100k_cw_2dB_traffic.json")
+    try:
+        t = json.load(open(path))
+    except (OSError, ValueError):
+        return None
+    if t.get("n_cw") != n or t.get("ebn0_db") != ebn0:
+        return None
+    return round(t["traffic_bytes_per_launch"])
+
+
 # ---------------------------------------------------------------------------
 # headline
 
@@ -484,7 +498,7 @@ def workload_headline(args) -> dict:
             "peak": round(VALU_PEAK / 1e12, 2),
             "unit": "Tops/s (VALU lane-ops)",
             "frac": round(achieved / VALU_PEAK, 4),
-            "traffic": None,
+            "traffic": measured_traffic(n, args.ebn0),
             "per_unit": {"lane_ops_per_guess": C_GUESS, "lane_ops_per_line_decode": C_LINE},
             "kernel_ms": round(kernel_s * 1e3, 2),
         },
