@@ -267,3 +267,15 @@ def test_graph_replays_match_eager(decoder, backend):
             eager.decode(other)
             hip.synchronize()
     decoder.set_params(use_graph=0)
+
+
+def test_flow_script_end_to_end(tmp_path):
+    # the reference pipeline's five steps with this decoder, at 3.5 dB
+    env = dict(os.environ, SNR_DB="3.5", N_CW="24", SEED="5")
+    r = subprocess.run(["bash", os.path.join(ROOT, "sogrand_amd", "cubic_flow.sh")], cwd=tmp_path,
+                       env=env, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = r.stdout
+    assert "Decoding complete. 24 block(s) decoded." in out
+    assert "SUCCESS: Files are identical." in out
+    assert (tmp_path / "decoded_cubic.bin").stat().st_size == 24 * 64
