@@ -50,8 +50,8 @@ extern "C" {
 #define SOGRAND_BACKEND_PERSISTENT 1 /* persistent waves + device queue          */
 
 /* Decoder parameters. Defaults (sogrand_params_default) are the reference's
- * constants: L=3, Imax=30, Tmax=UINT64_MAX, p_ET=1e-5, alpha=0.7
- * (SOGRAND_C/cubic_decoder_final.c:94-102). */
+ * constants: L=3, Imax=30, Tmax=UINT64_MAX, p_ET=1e-5, alpha=0.7, EPSILON=1e-40
+ * (SOGRAND_C/cubic_decoder_final.c:16, 94-102). */
 typedef struct sogrand_params {
     uint32_t L;         /* list size, 1..SOGRAND_MAX_L                                  */
     uint32_t Imax;      /* maximum product-code iterations (3 phases each), 1..1000     */
@@ -70,6 +70,9 @@ typedef struct sogrand_params {
     int32_t use_graph;  /* 1: when every buffer of a decode is device memory, capture its
                            launch sequence in a hipGraph once per (buffers, size,
                            parameters) and replay it on later calls                      */
+    double epsilon;     /* EPSILON of the list confidence test and the soft-output clamps:
+                           1e-40 as cubic_decoder_final.c:16 (default); DBL_EPSILON gives
+                           the reference's cubic_decoder_updated.c:16 variant            */
 } sogrand_params;
 
 /* Per-codeword statistics (the reference prints only their averages,

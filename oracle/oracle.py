@@ -16,6 +16,11 @@ ORACLE_SO = os.path.join(HERE, "liboracle.so")
 ORACLE_FMA_SO = os.path.join(HERE, "liboracle_fma.so")
 REF_SO = os.path.join(HERE, "_ref", "libsogrand_ref.so")
 REF_SO_NATIVE = os.path.join(HERE, "_ref", "libsogrand_ref_native.so")
+# the reference's EPSILON = DBL_EPSILON variant (cubic_decoder_updated.c)
+REF_SO_UPDATED = os.path.join(HERE, "_ref", "libsogrand_ref_updated.so")
+REF_BIN_UPDATED = os.path.join(HERE, "_ref", "cubic_decoder_updated")
+REF_BIN_UPDATED_NATIVE = os.path.join(HERE, "_ref", "cubic_decoder_updated_native")
+DBL_EPSILON = 2.220446049250313e-16
 REF_BIN_NATIVE = os.path.join(HERE, "_ref", "cubic_decoder_final_native")
 REF_BIN = os.path.join(HERE, "_ref", "cubic_decoder_final")
 REF_BIN_O2 = os.path.join(HERE, "_ref", "cubic_decoder_final_O2")
@@ -92,7 +97,13 @@ class Oracle:
                                              C.c_double, C.c_double, C.c_void_p, C.c_void_p,
                                              C.c_void_p, C.c_void_p, C.c_int]
         lib.og_decode_cubes_cond.restype = C.c_int
+        lib.og_set_epsilon.argtypes = [C.c_double]
+        lib.og_set_epsilon.restype = None
         self.lib = lib
+
+    def set_epsilon(self, eps: float) -> None:
+        """EPSILON of the soft output and confidence test, for every later call."""
+        self.lib.og_set_epsilon(eps)
 
     def code_tables(self):
         G = np.zeros((8, 16), np.uint8)

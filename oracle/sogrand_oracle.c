@@ -261,11 +261,17 @@ static int32_t og_cvtt_i32(double r)
     return INT32_MIN;
 }
 
+/* EPSILON of the soft output and the confidence test: 1e-40 in
+ * cubic_decoder_final.c:16, DBL_EPSILON in the reference's cubic_decoder_updated.c:16. */
+static double og_epsilon = 1e-40;
+
+void og_set_epsilon(double eps) { og_epsilon = eps; }
+
 int og_siso(const double *llr, int L, uint64_t Tmax, double thres, double *L_APP, double *L_E,
             og_line_trace *trace)
 {
     og_init();
-    const double eps = 1e-40; /* EPSILON (:16) */
+    const double eps = og_epsilon; /* EPSILON (:16) */
     if (L < 1) L = 1;
     if (L > OG_MAXL) L = OG_MAXL;
     if (Tmax == 0) Tmax = 0x7fffffff; /* :607 with Inf from :14 */

@@ -56,6 +56,11 @@ void og_sort_with_perm(double *a, uint8_t *perm, int n);
  * how many were written (the schedule holds 65535 nonzero patterns). */
 int og_schedule_prefix(int32_t ic, uint16_t *masks, int count);
 
+/* EPSILON of the soft output and the confidence test (default 1e-40, the value
+ * of cubic_decoder_final.c:16; DBL_EPSILON is cubic_decoder_updated.c's). Applies
+ * to every later decode in the process. */
+void og_set_epsilon(double eps);
+
 /* Component SISO decode of one line. Returns 0, or -1 on a non-finite input.
  * L_APP / L_E may be NULL; trace may be NULL. */
 int og_siso(const double *llr, int L, uint64_t Tmax, double thres, double *L_APP, double *L_E,

@@ -74,6 +74,7 @@ class Params(C.Structure):
         ("lane_batch", C.c_int32),
         ("step_len", C.c_int32),
         ("use_graph", C.c_int32),
+        ("epsilon", C.c_double),
     ]
 
 

@@ -8,6 +8,8 @@
 //   --tmax N            guess budget per component decode (default: unlimited)
 //   --precision P       fp64 (default, bit-exact with SOGRAND_C) | fp32
 //   --backend B         persistent (default) | plain
+//   --epsilon E         EPSILON of the soft output, e.g. 2.220446049250313e-16 for the
+//                       reference's cubic_decoder_updated.c (default 1e-40)
 //   --stats FILE        per-codeword statistics, 24-byte records
 //                       {u32 half_iters, u32 converged, u64 NG, u64 NG_p}
 //   --soft-out FILE     final L_APP, 4096 float64 per codeword
@@ -57,6 +59,8 @@ int main(int argc, char *argv[])
             if (v == "plain") p.backend = SOGRAND_BACKEND_PLAIN;
             else if (v == "persistent") p.backend = SOGRAND_BACKEND_PERSISTENT;
             else return usage(argv[0]);
+        } else if (a == "--epsilon") {
+            p.epsilon = std::strtod(need(), nullptr);
         } else if (a == "--stats") {
             stats_path = need();
         } else if (a == "--soft-out") {

@@ -79,6 +79,7 @@ void fill_job(CubeJob &job, const sogrand_ctx *ctx, const sogrand_params &p)
     job.L = p.L;
     job.Imax = p.Imax;
     job.f32 = p.precision == SOGRAND_FP32_MIXED;
+    job.eps = p.epsilon;
 }
 
 hipError_t launch_cubes(const sogrand_params &p, const LaunchPlan &lp, const CubeJob &job,
@@ -115,6 +116,8 @@ void sogrand_params_default(sogrand_params *p)
     p->waves_per_cu = 0;
     p->lane_batch = 0;
     p->step_len = 0;
+    p->use_graph = 0;
+    p->epsilon = 1e-40;
 }
 
 const char *sogrand_strerror(int code)
@@ -146,6 +149,7 @@ static int check_params(const sogrand_params *p)
         return SOGRAND_EINVAL;
     if (p->lane_batch < 0 || p->lane_batch > 64 || p->step_len < 0 || p->step_len > 65536)
         return SOGRAND_EINVAL;
+    if (!(p->epsilon > 0.0 && p->epsilon < 0.5)) return SOGRAND_EINVAL;
     return SOGRAND_OK;
 }
 
@@ -408,7 +412,7 @@ int sogrand_bitso_lines(sogrand_ctx *ctx, const double *llr, size_t n_lines, dou
         return SOGRAND_EHIP;
     if (sg::launch_lines(d_in, n_lines, d_la, d_le, ctx->nguess_stage.p, ctx->status.p,
                          ctx->d_sched, effective_tmax(ctx->p.Tmax), 1.0 - ctx->p.p_ET, ctx->p.L,
-                         ctx->p.precision == SOGRAND_FP32_MIXED, st) != hipSuccess)
+                         ctx->p.epsilon, ctx->p.precision == SOGRAND_FP32_MIXED, st) != hipSuccess)
         return SOGRAND_EHIP;
     if (hipStreamSynchronize(st) != hipSuccess) return SOGRAND_EHIP;
     if (L_APP && hipMemcpy(L_APP, d_la, n_lines * 16 * sizeof(double), hipMemcpyDefault) !=

@@ -36,6 +36,7 @@ struct DecodeCfg {
     const uint16_t *sched;  // kSchedTables x kSchedStride masks
     uint32_t L;             // list size 1..kMaxL
     const ExpEntry *exptab; // exp table in LDS (load_exp_table)
+    double eps;             // EPSILON of the confidence test and the soft output
 };
 
 // Fill the LDS exp table (first 32 threads) - the caller synchronises.
@@ -398,7 +399,7 @@ __device__ __forceinline__ bool list_hit(LineList &ls, double e, uint32_t code, 
     ls.ppos += e;
     const double pneg = 0.00390625 * pnot; // 2^-(n-k) * P_notGuess
     ls.pnl = pneg;
-    const double conf = (ls.ppos + pneg > 1e-40) ? ls.ppos / (ls.ppos + pneg) : 1.0;
+    const double conf = (ls.ppos + pneg > cfg.eps) ? ls.ppos / (ls.ppos + pneg) : 1.0;
     ls.code |= (uint64_t)code << (16 * ls.nl);
     ls.nl += 1;
     return conf > cfg.thres || (uint32_t)ls.nl == cfg.L;
@@ -595,10 +596,10 @@ __device__ __forceinline__ void line_search(const LDS &S, uint32_t hd,
 }
 
 // Soft output of position i (SOGRAND_bitSO :536-575) given the line input x.
+// eps: EPSILON of the clamps (1e-40, :16; DBL_EPSILON in cubic_decoder_updated.c:16).
 __device__ __forceinline__ double soft_app(const ListEntries &ls, int nl, double pnotin, int i,
-                                           double x)
+                                           double x, double eps)
 {
-    const double eps = 1e-40;
     double pp1 = 1.0 / (1.0 + exp(x));
     double pp0 = 1.0 - pp1;
     pp1 = fmin(fmax(pp1, eps), 1.0 - eps);

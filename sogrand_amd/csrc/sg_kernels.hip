@@ -91,7 +91,7 @@ __global__ __launch_bounds__(kPlainThreads) void plain_kernel(CubeJob job)
     load_exp_table(s_exp, t);
     __syncthreads();
     const LineLdsT<Rec, kPlainThreads> S(s_delta, s_hp, t);
-    const DecodeCfg cfg{job.tmax, job.thres, job.sched, job.L, s_exp};
+    const DecodeCfg cfg{job.tmax, job.thres, job.sched, job.L, s_exp, job.eps};
     double *le = job.le_scratch + (size_t)blockIdx.x * kCube;
 
     for (size_t cube = blockIdx.x; cube < job.n_cw; cube += gridDim.x) {
@@ -117,7 +117,7 @@ __global__ __launch_bounds__(kPlainThreads) void plain_kernel(CubeJob job)
                     ls.nl = 0;
                     ls.pnl = 0.0;
                 }
-                const double pnotin = fmax(ls.pnl, 1e-40);
+                const double pnotin = fmax(ls.pnl, cfg.eps);
                 ListEntries ent;
                 list_entries<P>(ls, S, hd, pm_hd, ic, cfg, ent);
 #pragma unroll 1
@@ -129,7 +129,7 @@ __global__ __launch_bounds__(kPlainThreads) void plain_kernel(CubeJob job)
                         la = xin;
                         lenew = 0.0;
                     } else {
-                        la = soft_app(ent, ls.nl, pnotin, i, xin);
+                        la = soft_app(ent, ls.nl, pnotin, i, xin, cfg.eps);
                         lenew = la - xin;
                     }
                     le[p] = lenew;
@@ -221,7 +221,7 @@ __global__ __launch_bounds__(256) void lines_kernel(const double *llr, size_t n,
         ls.nl = 0;
         ls.pnl = 0.0;
     }
-    const double pnotin = fmax(ls.pnl, 1e-40);
+    const double pnotin = fmax(ls.pnl, cfg.eps);
     ListEntries ent;
     list_entries<P>(ls, S, hd, pm_hd, ic, cfg, ent);
 #pragma unroll 1
@@ -232,7 +232,7 @@ __global__ __launch_bounds__(256) void lines_kernel(const double *llr, size_t n,
             a = xi;
             e = 0.0;
         } else {
-            a = soft_app(ent, ls.nl, pnotin, i, xi);
+            a = soft_app(ent, ls.nl, pnotin, i, xi, cfg.eps);
             e = a - xi;
         }
         if (la) la[line * 16 + i] = a;
@@ -331,10 +331,10 @@ hipError_t launch_plain(const CubeJob &job, int grid, hipStream_t st)
 
 hipError_t launch_lines(const double *llr, size_t n_lines, double *la, double *le,
                         uint32_t *nguess, int32_t *status, const uint16_t *sched, uint64_t tmax,
-                        double thres, uint32_t L, bool f32, hipStream_t st)
+                        double thres, uint32_t L, double eps, bool f32, hipStream_t st)
 {
     if (n_lines == 0) return hipSuccess;
-    const DecodeCfg cfg{tmax, thres, sched, L, nullptr};
+    const DecodeCfg cfg{tmax, thres, sched, L, nullptr, eps};
     const dim3 g((unsigned)((n_lines + 255) / 256)), b(256);
     const bool finite = tmax_binds(tmax);
 #define SG_LINES(FT, PP)                                                                      \

@@ -24,6 +24,7 @@ struct CubeJob {
     uint32_t L;
     uint32_t Imax;
     bool f32;                 // single-precision path metrics (PrecF32)
+    double eps;               // EPSILON of the confidence test and the soft output
 };
 
 // Whether a guess budget can stop a search. A line tests at most 65536 patterns
@@ -47,7 +48,7 @@ constexpr int kPersistentWaves = 4;
 // Line-level SISO (SOGRAND_bitSO on many lines).
 hipError_t launch_lines(const double *llr, size_t n_lines, double *la, double *le,
                         uint32_t *nguess, int32_t *status, const uint16_t *sched, uint64_t tmax,
-                        double thres, uint32_t L, bool f32, hipStream_t st);
+                        double thres, uint32_t L, double eps, bool f32, hipStream_t st);
 
 // Synthetic LLR generation on the device.
 hipError_t launch_gen(uint64_t seed, uint64_t cw_offset, size_t n_cw, const double *sigma,

@@ -152,7 +152,7 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))
     Rec *rec_d = &s_d[wave][0];
     uint16_t *rec_h = &s_h[wave][0];
     const LineLdsT<Rec, 64> L(rec_d, rec_h, lane);
-    const DecodeCfg cfg{job.tmax, job.thres, job.sched, job.L, s_exp};
+    const DecodeCfg cfg{job.tmax, job.thres, job.sched, job.L, s_exp, job.eps};
     const size_t wave_id = (size_t)blockIdx.x * kWaves + wave;
     uint32_t *hd_bits = &s_hd[wave][0][0];
     uint32_t *cnt = &s_cnt[wave][0][0];
@@ -345,7 +345,7 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))
                         la = xin;
                         lenew = 0.0;
                     } else {
-                        la = soft_app(ent, ol.nl, fmax(ol.pnl, 1e-40), i, xin);
+                        la = soft_app(ent, ol.nl, fmax(ol.pnl, cfg.eps), i, xin, cfg.eps);
                         lenew = la - xin;
                     }
                     le[p] = lenew;

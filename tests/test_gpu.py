@@ -279,3 +279,39 @@ def test_flow_script_end_to_end(tmp_path):
     assert "Decoding complete. 24 block(s) decoded." in out
     assert "SUCCESS: Files are identical." in out
     assert (tmp_path / "decoded_cubic.bin").stat().st_size == 24 * 64
+
+
+@pytest.mark.parametrize("backend", [0, 1], ids=["plain", "persistent"])
+def test_epsilon_variant_matches_reference_variant(decoder, oracle, backend):
+    # the reference's DBL_EPSILON variant (cubic_decoder_updated.c): lines within
+    # the chain bound, whole decodes bit-exact on bits and half-iterations and on
+    # the counts its two builds agree on
+    sg = _sg()
+    from oracle import DBL_EPSILON
+
+    g = golden("variant_eps.npz")
+    meta = json.load(open(os.path.join(GOLDEN, "variant_eps.json")))
+    decoder.set_params(backend=backend, epsilon=DBL_EPSILON)
+    try:
+        la, le, ng = decoder.bitso(g["llr"])
+        assert (ng == g["N_guess"]).all()
+        kappa = chain_cond(g["T"], g["pNL"], g["curL"])[:, None]
+        assert within_chain_bound(la, g["L_APP"], kappa).all()
+        assert within_chain_bound(le, g["L_E"], kappa).all()
+        for tag, m in meta["cubes"].items():
+            llr = sg.gen_llr(m["seed"], m["n"], ebn0_db=float(tag))
+            r = decoder.decode(llr)
+            assert (r.bits == g[f"bits_{tag}"]).all()
+            assert (r.stats["half_iters"] == g[f"half_iters_{tag}"]).all()
+            ref = np.zeros(m["n"], sg.CW_STATS_DTYPE)
+            fma = np.zeros(m["n"], sg.CW_STATS_DTYPE)
+            for f in ("half_iters", "NG", "NG_p"):
+                ref[f] = g[f"{f}_{tag}"]
+                fma[f] = g[f"{f}_fma_{tag}"]
+            stable = stable_codewords(ref, fma, g[f"bits_{tag}"], g[f"bits_fma_{tag}"])
+            _say(f"variant {tag} dB: builds agree on {int(stable.sum())} of {m['n']}")
+            assert stable.mean() >= 0.95
+            assert (r.stats["NG"][stable] == g[f"NG_{tag}"][stable]).all()
+            assert (r.stats["NG_p"][stable] == g[f"NG_p_{tag}"][stable]).all()
+    finally:
+        decoder.set_params(epsilon=1e-40)

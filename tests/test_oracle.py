@@ -198,3 +198,37 @@ def test_oracle_rejects_nonfinite(oracle):
     x[3] = np.nan
     with pytest.raises(ValueError):
         oracle.siso(x)
+
+
+def test_oracle_epsilon_variant_matches_reference_variant(oracle):
+    # the reference's DBL_EPSILON variant (cubic_decoder_updated.c), recorded by
+    # tests/golden/make_golden_variant.py: lines and whole decodes, bit for bit
+    from oracle import DBL_EPSILON
+
+    g = golden("variant_eps.npz")
+    try:
+        oracle.set_epsilon(DBL_EPSILON)
+        la = np.zeros_like(g["llr"])
+        le = np.zeros_like(g["llr"])
+        ng = np.zeros(len(la), np.int64)
+        for i, x in enumerate(g["llr"]):
+            la[i], le[i], tr = oracle.siso(x)
+            ng[i] = tr.T
+        assert (ng == g["N_guess"]).all()
+        assert np.array_equal(la, g["L_APP"]) and np.array_equal(le, g["L_E"])
+        import sogrand_amd as sg
+        import json
+        meta = json.load(open(os.path.join(GOLDEN, "variant_eps.json")))
+        for tag, m in meta["cubes"].items():
+            llr = sg.gen_llr(m["seed"], m["n"], ebn0_db=float(tag))
+            bits, st, soft = oracle.decode(llr, soft=True)
+            assert (bits == g[f"bits_{tag}"]).all()
+            assert (st["half_iters"] == g[f"half_iters_{tag}"]).all()
+            assert (st["NG"] == g[f"NG_{tag}"]).all() and (st["NG_p"] == g[f"NG_p_{tag}"]).all()
+            ns = g[f"soft_{tag}"].shape[0]
+            assert np.array_equal(soft[:ns], g[f"soft_{tag}"])
+    finally:
+        oracle.set_epsilon(1e-40)
+    # the fixture tells the two EPSILON values apart
+    la0 = np.array([oracle.siso(x)[0] for x in g["llr"][:200]])
+    assert not np.array_equal(la0, g["L_APP"][:200])
