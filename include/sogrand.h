@@ -45,6 +45,14 @@ extern "C" {
 #define SOGRAND_FP64 0       /* bit-exact with SOGRAND_C                              */
 #define SOGRAND_FP32_MIXED 1 /* FP32 sort + pattern metrics, FP64 list normalisation */
 
+/* decoder variant: which of the reference's SOGRAND_C decoders to reproduce */
+#define SOGRAND_VARIANT_FINAL 0  /* cubic_decoder_final.c (with `epsilon` = DBL_EPSILON:
+                                    cubic_decoder_updated.c)                          */
+#define SOGRAND_VARIANT_LEGACY 1 /* cubic_decoder.c, the earlier decoder: JacLog cut at
+                                    +-30, IC guard, P_notGuess never lowered (every
+                                    search fills its list), clamps 1e-9 and 1e-30;
+                                    FP64 only, `epsilon` unused                       */
+
 /* backend */
 #define SOGRAND_BACKEND_PLAIN 0      /* one workgroup per codeword, static lines */
 #define SOGRAND_BACKEND_PERSISTENT 1 /* persistent waves + device queue          */
@@ -73,6 +81,7 @@ typedef struct sogrand_params {
     double epsilon;     /* EPSILON of the list confidence test and the soft-output clamps:
                            1e-40 as cubic_decoder_final.c:16 (default); DBL_EPSILON gives
                            the reference's cubic_decoder_updated.c:16 variant            */
+    int32_t variant;    /* SOGRAND_VARIANT_* (default SOGRAND_VARIANT_FINAL)             */
 } sogrand_params;
 
 /* Per-codeword statistics (the reference prints only their averages,

@@ -21,6 +21,10 @@ REF_SO_UPDATED = os.path.join(HERE, "_ref", "libsogrand_ref_updated.so")
 REF_BIN_UPDATED = os.path.join(HERE, "_ref", "cubic_decoder_updated")
 REF_BIN_UPDATED_NATIVE = os.path.join(HERE, "_ref", "cubic_decoder_updated_native")
 DBL_EPSILON = 2.220446049250313e-16
+# the reference's earlier decoder (cubic_decoder.c)
+REF_SO_LEGACY = os.path.join(HERE, "_ref", "libsogrand_ref_legacy.so")
+REF_BIN_LEGACY = os.path.join(HERE, "_ref", "cubic_decoder_legacy")
+REF_BIN_LEGACY_NATIVE = os.path.join(HERE, "_ref", "cubic_decoder_legacy_native")
 REF_BIN_NATIVE = os.path.join(HERE, "_ref", "cubic_decoder_final_native")
 REF_BIN = os.path.join(HERE, "_ref", "cubic_decoder_final")
 REF_BIN_O2 = os.path.join(HERE, "_ref", "cubic_decoder_final_O2")
@@ -99,11 +103,17 @@ class Oracle:
         lib.og_decode_cubes_cond.restype = C.c_int
         lib.og_set_epsilon.argtypes = [C.c_double]
         lib.og_set_epsilon.restype = None
+        lib.og_set_variant.argtypes = [C.c_int]
+        lib.og_set_variant.restype = None
         self.lib = lib
 
     def set_epsilon(self, eps: float) -> None:
         """EPSILON of the soft output and confidence test, for every later call."""
         self.lib.og_set_epsilon(eps)
+
+    def set_variant(self, variant: int) -> None:
+        """0: cubic_decoder_final.c semantics; 1: the earlier cubic_decoder.c."""
+        self.lib.og_set_variant(variant)
 
     def code_tables(self):
         G = np.zeros((8, 16), np.uint8)

@@ -203,11 +203,11 @@ int og_schedule_prefix(int32_t ic, uint16_t *masks, int count)
 /* Line decode (SOGRAND_bitSO :521-585 and sogrand_main_logic :587-746).      */
 /* ------------------------------------------------------------------------- */
 
-/* :762 */
-static double og_jaclog(double x)
+/* :762; the earlier cubic_decoder.c:699 cuts at +-30 */
+static double og_jaclog(double x, double lim)
 {
-    if (x > 50.0) return x;
-    if (x < -50.0) return 0.0;
+    if (x > lim) return x;
+    if (x < -lim) return 0.0;
     return log(1.0 + exp(x));
 }
 
@@ -267,11 +267,22 @@ static double og_epsilon = 1e-40;
 
 void og_set_epsilon(double eps) { og_epsilon = eps; }
 
+static int og_variant = 0;
+
+void og_set_variant(int variant) { og_variant = variant; }
+
 int og_siso(const double *llr, int L, uint64_t Tmax, double thres, double *L_APP, double *L_E,
             og_line_trace *trace)
 {
     og_init();
+    const int legacy = og_variant == 1;
     const double eps = og_epsilon; /* EPSILON (:16) */
+    /* cubic_decoder.c: 1e-9 in getLConf (:710), the P_notinlist floor and the
+     * clamps (:503, :509-510); 1e-30 inside the logs (:533) */
+    const double eps_conf = legacy ? 1e-9 : eps;
+    const double eps_clamp = legacy ? 1e-9 : eps;
+    const double eps_log = legacy ? 1e-30 : eps;
+    const double jlim = legacy ? 30.0 : 50.0;
     if (L < 1) L = 1;
     if (L > OG_MAXL) L = OG_MAXL;
     if (Tmax == 0) Tmax = 0x7fffffff; /* :607 with Inf from :14 */
@@ -291,18 +302,19 @@ int og_siso(const double *llr, int L, uint64_t Tmax, double thres, double *L_APP
     }
     /* PM of the hard decision, natural order, before sorting (:617) */
     double pm_hd = 0.0;
-    for (int i = 0; i < 16; i++) pm_hd += og_jaclog(-a[i]);
+    for (int i = 0; i < 16; i++) pm_hd += og_jaclog(-a[i], jlim);
     og_sort_with_perm(a, perm, 16);
 
     /* IC (:619-622) */
     double beta = (a[7] - a[0]) / 7.0;
     int32_t ic = og_cvtt_i32(round(a[0] / beta - 1.0));
     if (ic < 0) ic = 0;
+    if (legacy && !(beta > 1e-9)) ic = 0; /* cubic_decoder.c:580-585 */
 
     double delta[16];
     uint8_t hp[16];
     for (int b = 0; b < 16; b++) {
-        delta[b] = og_jaclog(a[b]) - og_jaclog(-a[b]);
+        delta[b] = og_jaclog(a[b], jlim) - og_jaclog(-a[b], jlim);
         hp[b] = og_hcol[perm[b]];
     }
 
@@ -327,7 +339,7 @@ int og_siso(const double *llr, int L, uint64_t Tmax, double thres, double *L_APP
         for (int q = 0; q <= nl; q++) ppos += exp(-lpm[q]);                              \
         double pneg = 0.00390625 * pnot; /* 2^-(n-k) (:771) */                           \
         pnl = pneg;                                                                      \
-        double conf = (ppos + pneg > eps) ? ppos / (ppos + pneg) : 1.0;                  \
+        double conf = (ppos + pneg > eps_conf) ? ppos / (ppos + pneg) : 1.0;             \
         if (trace) {                                                                     \
             trace->pm[nl] = lpm[nl];                                                     \
             trace->conf[nl] = conf;                                                      \
@@ -341,7 +353,10 @@ int og_siso(const double *llr, int L, uint64_t Tmax, double thres, double *L_APP
 
     /* guess 1: the hard decision itself (:624-641) */
     T = 1;
-    pnot -= exp(-pm_hd);
+    if (legacy)
+        pnl = exp(-pm_hd); /* cubic_decoder.c:590, overwritten by getLConf at a hit */
+    else
+        pnot -= exp(-pm_hd);
     if (s0 == 0) OG_HIT(pm_hd, hd, 0);
 
     {
@@ -367,7 +382,10 @@ int og_siso(const double *llr, int L, uint64_t Tmax, double thres, double *L_APP
                     cw ^= 1u << perm[b];
                 }
                 T++;
-                pnot -= exp(-pm);
+                if (legacy)
+                    pnl = exp(-pm); /* cubic_decoder.c:638 */
+                else
+                    pnot -= exp(-pm);
                 if (syn == 0) OG_HIT(pm, cw, s.wt);
             }
         }
@@ -391,14 +409,14 @@ search_done:
             for (int i = 0; i < 16; i++) L_E[i] = 0.0;
         return 0;
     }
-    double pnotin = fmax(pnl, eps);
+    double pnotin = fmax(pnl, eps_clamp);
     double p[OG_MAXL];
     for (int j = 0; j < nl; j++) p[j] = exp(-lpm[j]);
     for (int i = 0; i < 16; i++) {
         double pp1 = 1.0 / (1.0 + exp(llr[i]));
         double pp0 = 1.0 - pp1;
-        pp1 = fmin(fmax(pp1, eps), 1.0 - eps);
-        pp0 = fmin(fmax(pp0, eps), 1.0 - eps);
+        pp1 = fmin(fmax(pp1, eps_clamp), 1.0 - eps_clamp);
+        pp0 = fmin(fmax(pp0, eps_clamp), 1.0 - eps_clamp);
         double p1 = 0.0, p0 = 0.0;
         for (int j = 0; j < nl; j++) {
             double c = (double)((lcw[j] >> i) & 1u);
@@ -407,7 +425,7 @@ search_done:
         }
         p0 += pnotin * pp0;
         p1 += pnotin * pp1;
-        la[i] = log(fmax(p0, eps)) - log(fmax(p1, eps));
+        la[i] = log(fmax(p0, eps_log)) - log(fmax(p1, eps_log));
     }
     if (L_APP) memcpy(L_APP, la, sizeof la);
     if (L_E)

@@ -61,6 +61,14 @@ int og_schedule_prefix(int32_t ic, uint16_t *masks, int count);
  * to every later decode in the process. */
 void og_set_epsilon(double eps);
 
+/* Decoder variant for every later decode in the process: 0 = the semantics of
+ * cubic_decoder_final.c (and of _updated.c with its EPSILON), 1 = the
+ * reference's earlier cubic_decoder.c (JacLog cut at +-30, IC guard
+ * beta > 1e-9, P_notGuess never decremented for this code so the search runs
+ * until the list is full, P_neg of the last guess's weight when no hit ends
+ * the search, clamps 1e-9 and 1e-30). */
+void og_set_variant(int variant);
+
 /* Component SISO decode of one line. Returns 0, or -1 on a non-finite input.
  * L_APP / L_E may be NULL; trace may be NULL. */
 int og_siso(const double *llr, int L, uint64_t Tmax, double thres, double *L_APP, double *L_E,

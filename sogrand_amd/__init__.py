@@ -32,6 +32,8 @@ EBUSY = -7
 
 FP64 = 0
 FP32_MIXED = 1
+VARIANT_FINAL = 0   # SOGRAND_C/cubic_decoder_final.c (and _updated.c via epsilon)
+VARIANT_LEGACY = 1  # SOGRAND_C/cubic_decoder.c, the earlier decoder (FP64 only)
 BACKEND_PLAIN = 0
 BACKEND_PERSISTENT = 1
 UINT64_MAX = (1 << 64) - 1
@@ -75,6 +77,7 @@ class Params(C.Structure):
         ("step_len", C.c_int32),
         ("use_graph", C.c_int32),
         ("epsilon", C.c_double),
+        ("variant", C.c_int32),
     ]
 
 

@@ -10,6 +10,9 @@
 //   --backend B         persistent (default) | plain
 //   --epsilon E         EPSILON of the soft output, e.g. 2.220446049250313e-16 for the
 //                       reference's cubic_decoder_updated.c (default 1e-40)
+//   --variant V         final (default: cubic_decoder_final.c) | legacy (the earlier
+//                       cubic_decoder.c, whose summary starts with one more line,
+//                       cubic_decoder.c:104)
 //   --stats FILE        per-codeword statistics, 24-byte records
 //                       {u32 half_iters, u32 converged, u64 NG, u64 NG_p}
 //   --soft-out FILE     final L_APP, 4096 float64 per codeword
@@ -61,6 +64,11 @@ int main(int argc, char *argv[])
             else return usage(argv[0]);
         } else if (a == "--epsilon") {
             p.epsilon = std::strtod(need(), nullptr);
+        } else if (a == "--variant") {
+            std::string v = need();
+            if (v == "final") p.variant = SOGRAND_VARIANT_FINAL;
+            else if (v == "legacy") p.variant = SOGRAND_VARIANT_LEGACY;
+            else return usage(argv[0]);
         } else if (a == "--stats") {
             stats_path = need();
         } else if (a == "--soft-out") {
@@ -106,6 +114,9 @@ int main(int argc, char *argv[])
         std::fprintf(stderr, "Error: %s\n", sogrand_strerror(rc));
         return 1;
     }
+    if (p.variant == SOGRAND_VARIANT_LEGACY)
+        std::printf("Decoding %s to %s using cubic code (n=%d, k=%d)...\n", in_path, out_path,
+                    SOGRAND_N, SOGRAND_K);
     const size_t chunk = 65536;
     std::vector<double> llr;
     std::vector<uint8_t> bits;

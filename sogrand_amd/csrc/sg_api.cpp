@@ -39,6 +39,21 @@ using sg::is_device_ptr;
 
 namespace sg {
 
+int arith_of(const sogrand_params &p)
+{
+    if (p.variant == SOGRAND_VARIANT_LEGACY) return kArithF64Legacy;
+    return p.precision == SOGRAND_FP32_MIXED ? kArithF32 : kArithF64;
+}
+
+// EPSILON of the confidence test and clamps, and the floor inside the soft
+// output's logs: `epsilon` for both (cubic_decoder_final.c:16, :573), or
+// cubic_decoder.c's 1e-9 (:503, :509-510, :710) and 1e-30 (:533).
+static void eps_of(const sogrand_params &p, double &eps, double &eps_log)
+{
+    eps = p.variant == SOGRAND_VARIANT_LEGACY ? 1e-9 : p.epsilon;
+    eps_log = p.variant == SOGRAND_VARIANT_LEGACY ? 1e-30 : p.epsilon;
+}
+
 LaunchPlan plan_launch(const sogrand_ctx *ctx, const sogrand_params &p, size_t n)
 {
     LaunchPlan lp{};
@@ -49,8 +64,7 @@ LaunchPlan plan_launch(const sogrand_ctx *ctx, const sogrand_params &p, size_t n
         // a persistent grid must be resident all at once: never more workgroups
         // per CU than the occupancy the kernel's registers and LDS allow
         const uint64_t tmax = effective_tmax(p.Tmax);
-        const int occ = persistent_blocks_per_cu(slots, tmax_binds(tmax),
-                                                 p.precision == SOGRAND_FP32_MIXED);
+        const int occ = persistent_blocks_per_cu(slots, tmax_binds(tmax), arith_of(p));
         int bpc = p.waves_per_cu > 0 ? (p.waves_per_cu + wpb - 1) / wpb : occ;
         if (bpc > occ) bpc = occ;
         int blocks = ctx->cu_count * bpc;
@@ -78,8 +92,8 @@ void fill_job(CubeJob &job, const sogrand_ctx *ctx, const sogrand_params &p)
     job.alpha = p.alpha;
     job.L = p.L;
     job.Imax = p.Imax;
-    job.f32 = p.precision == SOGRAND_FP32_MIXED;
-    job.eps = p.epsilon;
+    job.arith = arith_of(p);
+    eps_of(p, job.eps, job.eps_log);
 }
 
 hipError_t launch_cubes(const sogrand_params &p, const LaunchPlan &lp, const CubeJob &job,
@@ -118,6 +132,7 @@ void sogrand_params_default(sogrand_params *p)
     p->step_len = 0;
     p->use_graph = 0;
     p->epsilon = 1e-40;
+    p->variant = SOGRAND_VARIANT_FINAL;
 }
 
 const char *sogrand_strerror(int code)
@@ -150,6 +165,9 @@ static int check_params(const sogrand_params *p)
     if (p->lane_batch < 0 || p->lane_batch > 64 || p->step_len < 0 || p->step_len > 65536)
         return SOGRAND_EINVAL;
     if (!(p->epsilon > 0.0 && p->epsilon < 0.5)) return SOGRAND_EINVAL;
+    if (p->variant != SOGRAND_VARIANT_FINAL && p->variant != SOGRAND_VARIANT_LEGACY)
+        return SOGRAND_EINVAL;
+    if (p->variant == SOGRAND_VARIANT_LEGACY && p->precision != SOGRAND_FP64) return SOGRAND_EINVAL;
     return SOGRAND_OK;
 }
 
@@ -410,9 +428,11 @@ int sogrand_bitso_lines(sogrand_ctx *ctx, const double *llr, size_t n_lines, dou
     if (hipMemcpyAsync(d_in, llr, n_lines * 16 * sizeof(double), hipMemcpyDefault, st) !=
         hipSuccess)
         return SOGRAND_EHIP;
+    double eps, eps_log;
+    sg::eps_of(ctx->p, eps, eps_log);
     if (sg::launch_lines(d_in, n_lines, d_la, d_le, ctx->nguess_stage.p, ctx->status.p,
                          ctx->d_sched, effective_tmax(ctx->p.Tmax), 1.0 - ctx->p.p_ET, ctx->p.L,
-                         ctx->p.epsilon, ctx->p.precision == SOGRAND_FP32_MIXED, st) != hipSuccess)
+                         eps, eps_log, sg::arith_of(ctx->p), st) != hipSuccess)
         return SOGRAND_EHIP;
     if (hipStreamSynchronize(st) != hipSuccess) return SOGRAND_EHIP;
     if (L_APP && hipMemcpy(L_APP, d_la, n_lines * 16 * sizeof(double), hipMemcpyDefault) !=

@@ -7,8 +7,10 @@
 // record), so that lanes reading the same position hit consecutive addresses.
 //
 // Numerics follow SOGRAND_C/cubic_decoder_final.c exactly (citations are to
-// that file): every double is formed by the same operations in the same order;
-// the file is compiled with -ffp-contract=off so nothing is fused.
+// that file unless another is named): every double is formed by the same
+// operations in the same order; the file is compiled with -ffp-contract=off so
+// nothing is fused. The policy PrecF64Legacy follows the reference's earlier
+// SOGRAND_C/cubic_decoder.c instead.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -36,7 +38,9 @@ struct DecodeCfg {
     const uint16_t *sched;  // kSchedTables x kSchedStride masks
     uint32_t L;             // list size 1..kMaxL
     const ExpEntry *exptab; // exp table in LDS (load_exp_table)
-    double eps;             // EPSILON of the confidence test and the soft output
+    double eps;             // EPSILON of the confidence test, P_notinlist floor and clamps
+    double eps_log;         // floor inside the soft output's logs (eps, or 1e-30 for
+                            // cubic_decoder.c:533)
 };
 
 // Fill the LDS exp table (first 32 threads) - the caller synchronises.
@@ -82,13 +86,29 @@ __device__ __forceinline__ double exp_neg(double pm, const ExpEntry *tab)
 //            1 - found-mass chain (P_notGuess), PM of the hard decision, list
 //            weights (exp(-pm) of hits, recomputed in double), confidence and
 //            soft output stay in double.
+//   PrecF64Legacy: double, with the rules of the reference's earlier decoder
+//            SOGRAND_C/cubic_decoder.c: JacLog cut at +-30 (:699), IC = 0 when
+//            beta <= 1e-9 (:580-585), and P_notGuess never lowered for this
+//            code (:601, :650: only even codes subtract), so the confidence
+//            P_pos / (P_pos + 2^-8) stays below 1 - 1e-5 and every search runs
+//            until its list is full.
 struct PrecF64 {
     using Rec = double;
     static constexpr bool kF32 = false;
+    static constexpr bool kLegacy = false;
+    static constexpr double kJacLim = 50.0;
 };
 struct PrecF32 {
     using Rec = float;
     static constexpr bool kF32 = true;
+    static constexpr bool kLegacy = false;
+    static constexpr double kJacLim = 50.0;
+};
+struct PrecF64Legacy {
+    using Rec = double;
+    static constexpr bool kF32 = false;
+    static constexpr bool kLegacy = true;
+    static constexpr double kJacLim = 30.0;
 };
 
 // exp(-pm) of one guess in single precision (v_exp_f32 on -pm log2 e), widened.
@@ -124,19 +144,30 @@ __device__ __forceinline__ int lowest_bit(uint32_t x)
     return b;
 }
 
-// JacLog (:762).
-__device__ __forceinline__ double jaclog(double x)
+// JacLog (:762; cut at lim = 50, or 30 in cubic_decoder.c:699).
+__device__ __forceinline__ double jaclog(double x, double lim)
 {
-    if (x > 50.0) return x;
-    if (x < -50.0) return 0.0;
+    if (x > lim) return x;
+    if (x < -lim) return 0.0;
     return log(1.0 + exp(x));
 }
+
 
 // x86-64 cvttsd2si into a 32-bit register: NaN / out of range -> INT32_MIN.
 __device__ __forceinline__ int32_t cvtt_i32(double r)
 {
     if (r >= -2147483648.0 && r < 2147483648.0) return (int32_t)r;
     return INT32_MIN;
+}
+
+// IC = max(round(absL[0]/beta - 1), 0), beta = (absL[7]-absL[0])/7 (:619-622);
+// cubic_decoder.c:580-585 takes IC = 0 unless beta > 1e-9.
+__device__ __forceinline__ int32_t intercept(double a_first, double a_eighth, bool guard)
+{
+    const double beta = (a_eighth - a_first) / 7.0;
+    const int32_t rr = cvtt_i32(round(a_first / beta - 1.0));
+    const int32_t ic = rr > 0 ? rr : 0;
+    return (guard && !(beta > 1e-9)) ? 0 : ic;
 }
 
 // Ascending sort of the 16 magnitudes held in the slot's LDS records (d = key,
@@ -202,7 +233,7 @@ struct ListEntries {
 // order: the reference's quicksort decides the order among equal magnitudes
 // (replayed by quicksort_lds), then PM_HD in natural order and the sorted
 // records. Returns PM_HD.
-template <class LDS> __device__ __forceinline__ double setup_with_ties(const LDS S)
+template <class LDS> __device__ __forceinline__ double setup_with_ties(const LDS S, double jlim)
 {
 #pragma unroll 1
     for (int i = 0; i < 16; ++i) S.h(i) = (uint16_t)(hcol_of((unsigned)i) | (i << 8));
@@ -213,11 +244,12 @@ template <class LDS> __device__ __forceinline__ double setup_with_ties(const LDS
     for (int b = 0; b < 16; ++b) slot_of |= (uint64_t)b << (4 * (S.h(b) >> 8));
     double pm_hd = 0.0;
 #pragma unroll 1
-    for (int i = 0; i < 16; ++i) pm_hd += jaclog(-S.d((int)((slot_of >> (4 * i)) & 15u)));
+    for (int i = 0; i < 16; ++i)
+        pm_hd += jaclog(-S.d((int)((slot_of >> (4 * i)) & 15u)), jlim);
 #pragma unroll 1
     for (int b = 0; b < 16; ++b) {
         const double ab = S.d(b);
-        S.d(b) = jaclog(ab) - jaclog(-ab);
+        S.d(b) = jaclog(ab, jlim) - jaclog(-ab, jlim);
     }
     return pm_hd;
 }
@@ -266,11 +298,14 @@ template <class LDS> __device__ __forceinline__ void place_by_rank(const LDS &S,
 //      of the permutation in place;
 //   4. with ties, the reference's quicksort decides the order among equal
 //      magnitudes, so it is replayed on the records (quicksort_lds).
-// Returns false if an input is not finite.
+// Returns false if an input is not finite. legacy: cubic_decoder.c's JacLog
+// cut and IC guard.
 template <class Src, class LDS>
 __device__ __forceinline__ bool line_setup(const Src &src, const LDS &S, uint32_t &hd,
-                                           uint32_t &s0, double &pm_hd, int32_t &ic)
+                                           uint32_t &s0, double &pm_hd, int32_t &ic,
+                                           bool legacy)
 {
+    const double jlim = legacy ? 30.0 : 50.0;
     bool finite = true;
     hd = 0;
     s0 = 0;
@@ -303,11 +338,8 @@ __device__ __forceinline__ bool line_setup(const Src &src, const LDS &S, uint32_
         if (r == 0) a_first = ai;
         if (r == 7) a_eighth = ai;
     }
-    // IC = max(round(absL[0]/beta - 1), 0), beta = (absL[7]-absL[0])/7 (:619-622);
-    // the two order statistics do not depend on how ties are arranged
-    const double beta = (a_eighth - a_first) / 7.0;
-    const int32_t rr = cvtt_i32(round(a_first / beta - 1.0));
-    ic = rr > 0 ? rr : 0;
+    // the two order statistics of IC do not depend on how ties are arranged
+    ic = intercept(a_first, a_eighth, legacy);
 
     pm_hd = 0.0;
     if (!tie) {
@@ -315,13 +347,13 @@ __device__ __forceinline__ bool line_setup(const Src &src, const LDS &S, uint32_
 #pragma unroll 1
         for (int i = 0; i < 16; ++i) {
             const double ai = S.d(i);
-            const double jn = jaclog(-ai);
+            const double jn = jaclog(-ai, jlim);
             pm_hd += jn;
-            S.d(i) = jaclog(ai) - jn;
+            S.d(i) = jaclog(ai, jlim) - jn;
         }
         place_by_rank(S, rank);
     } else {
-        pm_hd = setup_with_ties(S);
+        pm_hd = setup_with_ties(S, jlim);
     }
     S.d(-1) = 0.0; // null record read by exhausted masks in search_run
     S.h(-1) = 0;
@@ -363,16 +395,14 @@ __device__ __forceinline__ bool line_setup_f32(const Src &src, const LDS &S,
         if (r == 0) a_first = ai;
         if (r == 7) a_eighth = ai;
     }
-    const double beta = ((double)a_eighth - (double)a_first) / 7.0;
-    const int32_t rr = cvtt_i32(round((double)a_first / beta - 1.0));
-    ic = rr > 0 ? rr : 0;
+    ic = intercept((double)a_first, (double)a_eighth, false);
     pm_hd = 0.0;
 #pragma unroll 1
     for (int i = 0; i < 16; ++i) {
         const double ai = fabs(src(i));
-        const double jn = jaclog(-ai);
+        const double jn = jaclog(-ai, 50.0);
         pm_hd += jn;
-        S.d(i) = (float)(jaclog(ai) - jn);
+        S.d(i) = (float)(jaclog(ai, 50.0) - jn);
     }
     place_by_rank(S, rank);
     S.d(-1) = 0.0f;
@@ -389,7 +419,7 @@ __device__ __forceinline__ bool line_setup_p(const Src &src, const LDS &S,
     if constexpr (P::kF32)
         return line_setup_f32(src, S, hd, s0, pm_hd, ic);
     else
-        return line_setup(src, S, hd, s0, pm_hd, ic);
+        return line_setup(src, S, hd, s0, pm_hd, ic, P::kLegacy);
 }
 
 // Record a list entry and apply the stopping rule (:628-641, :674-686, getLConf :767-774).
@@ -432,6 +462,19 @@ __device__ __forceinline__ void list_entry(const LineList &ls, int q, const LDS 
     e = exp_neg((double)pm, cfg.exptab);
 }
 
+// Path metric of a pattern (mask over sorted positions) with the search's
+// arithmetic: the hard decision's metric plus the increments in ascending
+// sorted position (getPM, :765).
+template <class P, class LDS>
+__device__ __forceinline__ double pattern_pm(const LDS &S, uint32_t m, double pm_hd)
+{
+    using Rec = typename P::Rec;
+    Rec pm = (Rec)pm_hd;
+#pragma unroll 1
+    for (uint32_t r = m; r; r &= r - 1u) pm = pm + S.d(__builtin_ctz(r));
+    return (double)pm;
+}
+
 template <class P, class LDS>
 __device__ __forceinline__ void list_entries(const LineList &ls, const LDS &S, uint32_t hd,
                                              double pm_hd, int32_t ic, const DecodeCfg &cfg,
@@ -469,6 +512,7 @@ struct SearchState {
 };
 
 // Hard-decision guess (:624-641) and schedule selection.
+template <class P>
 __device__ __forceinline__ void search_init(SearchState &ss, LineList &ls, uint32_t hd, uint32_t s0,
                                             double pm_hd, int32_t ic, const DecodeCfg &cfg)
 {
@@ -478,7 +522,14 @@ __device__ __forceinline__ void search_init(SearchState &ss, LineList &ls, uint3
     ls.code = 0;
     ss.T = 1;
     const double e_hd = exp_neg(pm_hd, cfg.exptab);
-    ss.pnot = 1.0 - e_hd;
+    if (P::kLegacy) {
+        // cubic_decoder.c:590: P_notGuess stays 1; P_neg holds the last
+        // guess's weight until a hit replaces it
+        ss.pnot = 1.0;
+        ls.pnl = e_hd;
+    } else {
+        ss.pnot = 1.0 - e_hd;
+    }
     ss.done = false;
     if (s0 == 0) ss.done = list_hit(ls, e_hd, 0u, ss.pnot, cfg);
     const int tbl = ic < kSchedGenericIc ? ic : kSchedGenericIc;
@@ -496,6 +547,12 @@ __device__ __forceinline__ void search_init(SearchState &ss, LineList &ls, uint3
 // load with the next load issued one chunk ahead. The loop runs a step count
 // common to the wave and leaves when no lane is still searching; a finished
 // lane idles inside it, which keeps the wave's control flow to one mask.
+//
+// In the earlier decoder (P::kLegacy) P_notGuess never changes, so a pattern's
+// metric matters only if it is a codeword or, under a guess budget, the last
+// guess (whose weight is the P_neg a budget stop leaves, cubic_decoder.c:638).
+// Without a budget the loop forms syndromes alone and a hit's metric is summed
+// from its mask afterwards, with the same additions.
 template <bool FINITE_T, class P, class LDS>
 __device__ __forceinline__ void search_run(SearchState &ss, LineList &ls, const LDS &S, uint32_t hd,
                                            uint32_t s0, double pm_hd, int32_t ic,
@@ -503,6 +560,7 @@ __device__ __forceinline__ void search_run(SearchState &ss, LineList &ls, const 
 {
     using Rec = typename P::Rec;
     static_assert(sizeof(Rec) == sizeof(typename LDS::Rec), "record type");
+    constexpr bool kPmLoop = !(P::kLegacy && !FINITE_T);
     const Rec pm_base = (Rec)pm_hd;
 #pragma unroll 1
     for (uint32_t step = 0; step < max_steps; step += 2) {
@@ -546,30 +604,36 @@ __device__ __forceinline__ void search_run(SearchState &ss, LineList &ls, const 
             const uint32_t q1 = r1 & (r1 - 1u), q2 = r2 & (r2 - 1u);
             const int b1 = lowest_bit(r1), c1 = lowest_bit(q1);
             const int b2 = lowest_bit(r2), c2 = lowest_bit(q2);
-            const Rec d1 = S.d(b1), e1 = S.d(c1), d2 = S.d(b2), e2 = S.d(c2);
             syn1 ^= (uint32_t)S.h(b1) ^ (uint32_t)S.h(c1);
             syn2 ^= (uint32_t)S.h(b2) ^ (uint32_t)S.h(c2);
-            pm1 = (pm1 + d1) + e1;
-            pm2 = (pm2 + d2) + e2;
+            if constexpr (kPmLoop) {
+                const Rec d1 = S.d(b1), e1 = S.d(c1), d2 = S.d(b2), e2 = S.d(c2);
+                pm1 = (pm1 + d1) + e1;
+                pm2 = (pm2 + d2) + e2;
+            }
             r1 = q1 & (q1 - 1u);
             r2 = q2 & (q2 - 1u);
         } while (r1 | r2);
         const bool hit1 = (syn1 & 0xffu) == 0, hit2 = (syn2 & 0xffu) == 0;
-        double e1, e2;
-        if (P::kF32) {
+        double e1 = 0.0, e2 = 0.0;
+        if constexpr (P::kF32) {
             // single-precision weights for the found-mass chain; a hit's list
             // weight is taken in double from its (single-precision) metric,
             // in the (rare) hit branch below
             e1 = exp_neg_f32((float)pm1);
             e2 = exp_neg_f32((float)pm2);
-        } else {
+        } else if constexpr (kPmLoop) {
             e1 = exp_neg((double)pm1, cfg.exptab);
             e2 = exp_neg((double)pm2, cfg.exptab);
         }
         ss.T += 1;
         ss.t = t + 1;
         if (P::kF32 && hit1) e1 = exp_neg((double)pm1, cfg.exptab);
-        ss.pnot -= e1;
+        if (!kPmLoop && hit1) e1 = exp_neg(pattern_pm<P>(S, m1, pm_hd), cfg.exptab);
+        if constexpr (!P::kLegacy)
+            ss.pnot -= e1;
+        else if (kPmLoop && !hit1)
+            ls.pnl = e1; // cubic_decoder.c:638
         bool stop = hit1 && list_hit(ls, e1, t + 1u, ss.pnot, cfg);
         // end of the schedule, or the next weight level is over budget
         if (!stop && !take2) stop = true;
@@ -577,7 +641,11 @@ __device__ __forceinline__ void search_run(SearchState &ss, LineList &ls, const 
             ss.T += 1;
             ss.t = t + 2;
             if (P::kF32 && hit2) e2 = exp_neg((double)pm2, cfg.exptab);
-            ss.pnot -= e2;
+            if (!kPmLoop && hit2) e2 = exp_neg(pattern_pm<P>(S, m2, pm_hd), cfg.exptab);
+            if constexpr (!P::kLegacy)
+                ss.pnot -= e2;
+            else if (kPmLoop && !hit2)
+                ls.pnl = e2;
             stop = hit2 && list_hit(ls, e2, t + 2u, ss.pnot, cfg);
         }
         ss.done = stop;
@@ -590,15 +658,17 @@ __device__ __forceinline__ void line_search(const LDS &S, uint32_t hd,
                                             const DecodeCfg &cfg, LineList &ls)
 {
     SearchState ss;
-    search_init(ss, ls, hd, s0, pm_hd, ic, cfg);
+    search_init<P>(ss, ls, hd, s0, pm_hd, ic, cfg);
     search_run<FINITE_T, P>(ss, ls, S, hd, s0, pm_hd, ic, cfg, 0xffffffffu);
     ls.T = ss.T;
 }
 
 // Soft output of position i (SOGRAND_bitSO :536-575) given the line input x.
-// eps: EPSILON of the clamps (1e-40, :16; DBL_EPSILON in cubic_decoder_updated.c:16).
+// eps: EPSILON of the clamps (1e-40, :16; DBL_EPSILON in cubic_decoder_updated.c:16;
+// 1e-9 in cubic_decoder.c:509-510); eps_log: floor inside the logs (EPSILON, or
+// 1e-30 in cubic_decoder.c:533).
 __device__ __forceinline__ double soft_app(const ListEntries &ls, int nl, double pnotin, int i,
-                                           double x, double eps)
+                                           double x, double eps, double eps_log)
 {
     double pp1 = 1.0 / (1.0 + exp(x));
     double pp0 = 1.0 - pp1;
@@ -614,7 +684,7 @@ __device__ __forceinline__ double soft_app(const ListEntries &ls, int nl, double
         }
     p0 += pnotin * pp0;
     p1 += pnotin * pp1;
-    return log(fmax(p0, eps)) - log(fmax(p1, eps));
+    return log(fmax(p0, eps_log)) - log(fmax(p1, eps_log));
 }
 
 } // namespace sg

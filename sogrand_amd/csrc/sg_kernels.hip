@@ -91,7 +91,7 @@ __global__ __launch_bounds__(kPlainThreads) void plain_kernel(CubeJob job)
     load_exp_table(s_exp, t);
     __syncthreads();
     const LineLdsT<Rec, kPlainThreads> S(s_delta, s_hp, t);
-    const DecodeCfg cfg{job.tmax, job.thres, job.sched, job.L, s_exp, job.eps};
+    const DecodeCfg cfg{job.tmax, job.thres, job.sched, job.L, s_exp, job.eps, job.eps_log};
     double *le = job.le_scratch + (size_t)blockIdx.x * kCube;
 
     for (size_t cube = blockIdx.x; cube < job.n_cw; cube += gridDim.x) {
@@ -129,7 +129,7 @@ __global__ __launch_bounds__(kPlainThreads) void plain_kernel(CubeJob job)
                         la = xin;
                         lenew = 0.0;
                     } else {
-                        la = soft_app(ent, ls.nl, pnotin, i, xin, cfg.eps);
+                        la = soft_app(ent, ls.nl, pnotin, i, xin, cfg.eps, cfg.eps_log);
                         lenew = la - xin;
                     }
                     le[p] = lenew;
@@ -232,7 +232,7 @@ __global__ __launch_bounds__(256) void lines_kernel(const double *llr, size_t n,
             a = xi;
             e = 0.0;
         } else {
-            a = soft_app(ent, ls.nl, pnotin, i, xi, cfg.eps);
+            a = soft_app(ent, ls.nl, pnotin, i, xi, cfg.eps, cfg.eps_log);
             e = a - xi;
         }
         if (la) la[line * 16 + i] = a;
@@ -319,28 +319,33 @@ hipError_t launch_plain(const CubeJob &job, int grid, hipStream_t st)
     if (job.n_cw == 0) return hipSuccess;
     const bool finite = tmax_binds(job.tmax);
     const dim3 g(grid), b(kPlainThreads);
-    if (job.f32) {
-        if (finite) hipLaunchKernelGGL((plain_kernel<true, PrecF32>), g, b, 0, st, job);
-        else hipLaunchKernelGGL((plain_kernel<false, PrecF32>), g, b, 0, st, job);
+#define SG_PLAIN(FT, PP) hipLaunchKernelGGL((plain_kernel<FT, PP>), g, b, 0, st, job)
+    if (job.arith == kArithF32) {
+        if (finite) SG_PLAIN(true, PrecF32); else SG_PLAIN(false, PrecF32);
+    } else if (job.arith == kArithF64Legacy) {
+        if (finite) SG_PLAIN(true, PrecF64Legacy); else SG_PLAIN(false, PrecF64Legacy);
     } else {
-        if (finite) hipLaunchKernelGGL((plain_kernel<true, PrecF64>), g, b, 0, st, job);
-        else hipLaunchKernelGGL((plain_kernel<false, PrecF64>), g, b, 0, st, job);
+        if (finite) SG_PLAIN(true, PrecF64); else SG_PLAIN(false, PrecF64);
     }
+#undef SG_PLAIN
     return hipGetLastError();
 }
 
 hipError_t launch_lines(const double *llr, size_t n_lines, double *la, double *le,
                         uint32_t *nguess, int32_t *status, const uint16_t *sched, uint64_t tmax,
-                        double thres, uint32_t L, double eps, bool f32, hipStream_t st)
+                        double thres, uint32_t L, double eps, double eps_log, int arith,
+                        hipStream_t st)
 {
     if (n_lines == 0) return hipSuccess;
-    const DecodeCfg cfg{tmax, thres, sched, L, nullptr, eps};
+    const DecodeCfg cfg{tmax, thres, sched, L, nullptr, eps, eps_log};
     const dim3 g((unsigned)((n_lines + 255) / 256)), b(256);
     const bool finite = tmax_binds(tmax);
 #define SG_LINES(FT, PP)                                                                      \
     hipLaunchKernelGGL((lines_kernel<FT, PP>), g, b, 0, st, llr, n_lines, la, le, nguess, status, cfg)
-    if (f32) {
+    if (arith == kArithF32) {
         if (finite) SG_LINES(true, PrecF32); else SG_LINES(false, PrecF32);
+    } else if (arith == kArithF64Legacy) {
+        if (finite) SG_LINES(true, PrecF64Legacy); else SG_LINES(false, PrecF64Legacy);
     } else {
         if (finite) SG_LINES(true, PrecF64); else SG_LINES(false, PrecF64);
     }

@@ -8,6 +8,11 @@
 
 namespace sg {
 
+// Arithmetic policy of the line decoder (sg_device.h): the reference's
+// cubic_decoder_final.c in double (PrecF64) or with single-precision path
+// metrics (PrecF32), or its earlier cubic_decoder.c in double (PrecF64Legacy).
+enum { kArithF64 = 0, kArithF32 = 1, kArithF64Legacy = 2 };
+
 struct CubeJob {
     const double *llr;        // n_cw x 4096 (device)
     uint8_t *info;            // n_cw x 64 (device)
@@ -23,8 +28,9 @@ struct CubeJob {
     double alpha;
     uint32_t L;
     uint32_t Imax;
-    bool f32;                 // single-precision path metrics (PrecF32)
-    double eps;               // EPSILON of the confidence test and the soft output
+    int arith;                // kArithF64 / kArithF32 / kArithF64Legacy
+    double eps;               // EPSILON of the confidence test, P_notinlist floor and clamps
+    double eps_log;           // floor inside the soft output's logs
 };
 
 // Whether a guess budget can stop a search. A line tests at most 65536 patterns
@@ -42,13 +48,14 @@ size_t plain_scratch_slots(size_t n_cw);
 hipError_t launch_persistent(const CubeJob &job, int waves_per_block, int blocks, int slots,
                              uint32_t lane_batch, uint32_t step_len, hipStream_t st);
 // Resident workgroups per CU of the persistent kernel (occupancy API).
-int persistent_blocks_per_cu(int slots, bool finite, bool f32);
+int persistent_blocks_per_cu(int slots, bool finite, int arith);
 constexpr int kPersistentWaves = 4;
 
 // Line-level SISO (SOGRAND_bitSO on many lines).
 hipError_t launch_lines(const double *llr, size_t n_lines, double *la, double *le,
                         uint32_t *nguess, int32_t *status, const uint16_t *sched, uint64_t tmax,
-                        double thres, uint32_t L, double eps, bool f32, hipStream_t st);
+                        double thres, uint32_t L, double eps, double eps_log, int arith,
+                        hipStream_t st);
 
 // Synthetic LLR generation on the device.
 hipError_t launch_gen(uint64_t seed, uint64_t cw_offset, size_t n_cw, const double *sigma,

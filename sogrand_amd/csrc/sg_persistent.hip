@@ -152,7 +152,7 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))
     Rec *rec_d = &s_d[wave][0];
     uint16_t *rec_h = &s_h[wave][0];
     const LineLdsT<Rec, 64> L(rec_d, rec_h, lane);
-    const DecodeCfg cfg{job.tmax, job.thres, job.sched, job.L, s_exp, job.eps};
+    const DecodeCfg cfg{job.tmax, job.thres, job.sched, job.L, s_exp, job.eps, job.eps_log};
     const size_t wave_id = (size_t)blockIdx.x * kWaves + wave;
     uint32_t *hd_bits = &s_hd[wave][0][0];
     uint32_t *cnt = &s_cnt[wave][0][0];
@@ -345,7 +345,8 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))
                         la = xin;
                         lenew = 0.0;
                     } else {
-                        la = soft_app(ent, ol.nl, fmax(ol.pnl, cfg.eps), i, xin, cfg.eps);
+                        la = soft_app(ent, ol.nl, fmax(ol.pnl, cfg.eps), i, xin, cfg.eps,
+                                      cfg.eps_log);
                         lenew = la - xin;
                     }
                     le[p] = lenew;
@@ -417,8 +418,8 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))
                         rank += (aj < key || (aj == key && j < i)) ? 1u : 0u;
                         if (!P::kF32) tie = tie || (aj == key && j != i);
                     }
-                    const double jn = jaclog(-ai);
-                    delta = jaclog(ai) - jn; // getPM's increment (:765)
+                    const double jn = jaclog(-ai, P::kJacLim);
+                    delta = jaclog(ai, P::kJacLim) - jn; // getPM's increment (:765)
                     stage_jn[lane] = jn;
                     // the two order statistics of IC do not depend on tie order
                     if (rank == 0) stage_a[2 * q] = (double)key;
@@ -440,10 +441,7 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))
 #pragma unroll 1
                     for (int j = 0; j < 16; ++j) pm_hd += stage_jn[16 * myq + j]; // getPM_HD (:617)
                     // IC (:619-622)
-                    const double a_first = stage_a[2 * myq], a_eighth = stage_a[2 * myq + 1];
-                    const double beta = (a_eighth - a_first) / 7.0;
-                    const int32_t rr = cvtt_i32(round(a_first / beta - 1.0));
-                    ic = rr > 0 ? rr : 0;
+                    ic = intercept(stage_a[2 * myq], stage_a[2 * myq + 1], P::kLegacy);
                 }
                 // tasks of ordinary lines write the sorted records
                 if (owner >= 0 && ((special >> (16 * q)) & 0xffffull) == 0) {
@@ -465,9 +463,9 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))
                     } else {
                         if constexpr (!P::kF32) {
                             // records hold |x| in natural order
-                            if (line_special) pm_hd = setup_with_ties(L);
+                            if (line_special) pm_hd = setup_with_ties(L, P::kJacLim);
                         }
-                        search_init(ss, ls, hd, s0, pm_hd, ic, cfg);
+                        search_init<P>(ss, ls, hd, s0, pm_hd, ic, cfg);
                         state = ss.done ? kDone : kSearch;
                     }
                 }
@@ -481,7 +479,7 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))
     }
 }
 
-int persistent_blocks_per_cu(int slots, bool finite, bool f32)
+int persistent_blocks_per_cu(int slots, bool finite, int arith)
 {
     int nb = 0;
     hipError_t e = hipErrorInvalidValue;
@@ -489,7 +487,9 @@ int persistent_blocks_per_cu(int slots, bool finite, bool f32)
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, persistent_kernel<FT, SS, PP>,       \
                                                      64 * kWaves, 0)
 #define SG_OCC_P(FT, SS)                                                                      \
-    if (f32) SG_OCC(FT, SS, PrecF32); else SG_OCC(FT, SS, PrecF64)
+    if (arith == kArithF32) SG_OCC(FT, SS, PrecF32);                                          \
+    else if (arith == kArithF64Legacy) SG_OCC(FT, SS, PrecF64Legacy);                         \
+    else SG_OCC(FT, SS, PrecF64)
     if (slots == 1) {
         if (finite) { SG_OCC_P(true, 1); } else { SG_OCC_P(false, 1); }
     } else if (slots == 2) {
@@ -519,7 +519,9 @@ hipError_t launch_persistent(const CubeJob &job, int waves_per_block, int blocks
 #define SG_LAUNCH(FT, SS, PP)                                                                 \
     hipLaunchKernelGGL((persistent_kernel<FT, SS, PP>), g, b, 0, st, job, thresh, step)
 #define SG_LAUNCH_P(FT, SS)                                                                   \
-    if (job.f32) SG_LAUNCH(FT, SS, PrecF32); else SG_LAUNCH(FT, SS, PrecF64)
+    if (job.arith == kArithF32) SG_LAUNCH(FT, SS, PrecF32);                                   \
+    else if (job.arith == kArithF64Legacy) SG_LAUNCH(FT, SS, PrecF64Legacy);                  \
+    else SG_LAUNCH(FT, SS, PrecF64)
     if (slots == 1) {
         if (finite) { SG_LAUNCH_P(true, 1); } else { SG_LAUNCH_P(false, 1); }
     } else if (slots == 2) {

@@ -237,6 +237,28 @@ def test_cli_drop_in(tmp_path, oracle):
     assert np.fromfile(fst, sg.CW_STATS_DTYPE).tolist() == st.tolist()
 
 
+def test_cli_legacy_variant(tmp_path):
+    # `--variant legacy` stands in for the reference's cubic_decoder.c binary:
+    # its extra first summary line, and the fixture's bits and counts
+    sg = _sg()
+    g = golden("variant_legacy.npz")
+    m = json.load(open(os.path.join(GOLDEN, "variant_legacy.json")))["cubes"]["3.0"]
+    llr = sg.gen_llr(m["seed"], m["n"], ebn0_db=3.0)
+    fin, fout = tmp_path / "llr.bin", tmp_path / "dec.bin"
+    llr.astype("<f8").tofile(fin)
+    r = subprocess.run([sg.CLI_PATH, str(fin), str(fout), "--variant", "legacy"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert np.array_equal(np.fromfile(fout, np.uint8).reshape(-1, 64), g["bits_3.0"])
+    n = m["n"]
+    lines = r.stdout.splitlines()
+    assert lines[0] == f"Decoding {fin} to {fout} using cubic code (n=16, k=8)..."
+    assert lines[1] == f"Decoding complete. {n} block(s) decoded."
+    assert lines[2] == f"Average iterations per block: {g['half_iters_3.0'].sum() / 2 / n:.2f}"
+    assert lines[3] == f"Average NG per block: {g['NG_3.0'].sum() / n:.2f}"
+    assert lines[5] == f"Average NG_p per block: {g['NG_p_3.0'].sum() / n:.2f}"
+
+
 @pytest.mark.parametrize("backend", [0, 1], ids=["plain", "persistent"])
 def test_graph_replays_match_eager(decoder, backend):
     # a captured launch sequence replayed several times, with eager decodes on
@@ -315,3 +337,54 @@ def test_epsilon_variant_matches_reference_variant(decoder, oracle, backend):
             assert (r.stats["NG_p"][stable] == g[f"NG_p_{tag}"][stable]).all()
     finally:
         decoder.set_params(epsilon=1e-40)
+
+
+@pytest.mark.parametrize("backend", [0, 1], ids=["plain", "persistent"])
+def test_legacy_variant_matches_reference_legacy(decoder, oracle, backend):
+    # the reference's earlier decoder (cubic_decoder.c), recorded by
+    # tests/golden/make_golden_variant.py legacy. It has no found-mass chain, so
+    # every count is fixed by its arithmetic: guesses, bits, half-iterations, NG
+    # and NG_p exact everywhere (its two builds agree on all of them). Soft
+    # outputs are logs of sums of positive terms: a component decode within
+    # 1e-12 absolute, a whole decode's final L_APP within 1e-10 relative
+    # (1 absolute floor); both are about 1e-14 on MI355X.
+    sg = _sg()
+    g = golden("variant_legacy.npz")
+    meta = json.load(open(os.path.join(GOLDEN, "variant_legacy.json")))
+    decoder.set_params(backend=backend, variant=sg.VARIANT_LEGACY)
+    try:
+        la, le, ng = decoder.bitso(g["llr"])
+        assert (ng == g["N_guess"]).all()
+        assert np.abs(la - g["L_APP"]).max() <= 1e-12
+        assert np.abs(le - g["L_E"]).max() <= 1e-12
+        # a guess budget: the last guess's weight is the P_neg left behind
+        decoder.set_params(Tmax=64)
+        la, le, ng = decoder.bitso(g["llr"])
+        assert (ng == g["N_guess_t64"]).all()
+        assert np.abs(la - g["L_APP_t64"]).max() <= 1e-12
+        assert np.abs(le - g["L_E_t64"]).max() <= 1e-12
+        llr = sg.gen_llr(7001, 8, ebn0_db=3.0)
+        r = decoder.decode(llr)
+        oracle.set_variant(1)
+        try:
+            bits, st, _ = oracle.decode(llr, Tmax=64)
+        finally:
+            oracle.set_variant(0)
+        assert (r.bits == bits).all()
+        for f in ("half_iters", "NG", "NG_p"):
+            assert (r.stats[f] == st[f]).all(), f
+        decoder.set_params(Tmax=sg.UINT64_MAX)
+        for tag, m in meta["cubes"].items():
+            _say(f"legacy {tag} dB, backend {backend}")
+            llr = sg.gen_llr(m["seed"], m["n"], ebn0_db=float(tag))
+            r = decoder.decode(llr, soft=True)
+            assert (r.bits == g[f"bits_{tag}"]).all()
+            for f in ("half_iters", "NG", "NG_p"):
+                assert (r.stats[f] == g[f"{f}_{tag}"]).all(), f
+            ref = g[f"soft_{tag}"]
+            ns = ref.shape[0]
+            assert (np.abs(r.soft[:ns] - ref) <= 1e-10 * np.maximum(1.0, np.abs(ref))).all()
+        with pytest.raises(Exception):
+            decoder.set_params(precision=sg.FP32_MIXED)  # the earlier decoder is FP64 only
+    finally:
+        decoder.set_params(variant=sg.VARIANT_FINAL, Tmax=sg.UINT64_MAX, precision=sg.FP64)

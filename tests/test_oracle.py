@@ -232,3 +232,46 @@ def test_oracle_epsilon_variant_matches_reference_variant(oracle):
     # the fixture tells the two EPSILON values apart
     la0 = np.array([oracle.siso(x)[0] for x in g["llr"][:200]])
     assert not np.array_equal(la0, g["L_APP"][:200])
+
+
+def test_oracle_legacy_variant_matches_reference_legacy(oracle):
+    # the reference's earlier decoder (cubic_decoder.c), recorded by
+    # tests/golden/make_golden_variant.py legacy: lines and whole decodes, bit for bit
+    import json
+
+    import sogrand_amd as sg
+
+    g = golden("variant_legacy.npz")
+    try:
+        oracle.set_variant(1)
+        la = np.zeros_like(g["llr"])
+        le = np.zeros_like(g["llr"])
+        ng = np.zeros(len(la), np.int64)
+        for i, x in enumerate(g["llr"]):
+            la[i], le[i], tr = oracle.siso(x)
+            ng[i] = tr.T
+            assert tr.n_list == g["curL"][i]
+        assert (ng == g["N_guess"]).all()
+        assert np.array_equal(la, g["L_APP"]) and np.array_equal(le, g["L_E"])
+        # under a guess budget the last guess's weight is the P_neg left behind
+        for i, x in enumerate(g["llr"]):
+            la[i], le[i], tr = oracle.siso(x, Tmax=64)
+            ng[i] = tr.T
+        assert (ng == g["N_guess_t64"]).all()
+        assert np.array_equal(la, g["L_APP_t64"]) and np.array_equal(le, g["L_E_t64"])
+        meta = json.load(open(os.path.join(GOLDEN, "variant_legacy.json")))
+        for tag, m in meta["cubes"].items():
+            llr = sg.gen_llr(m["seed"], m["n"], ebn0_db=float(tag))
+            bits, st, soft = oracle.decode(llr, soft=True)
+            assert (bits == g[f"bits_{tag}"]).all()
+            assert (st["half_iters"] == g[f"half_iters_{tag}"]).all()
+            assert (st["NG"] == g[f"NG_{tag}"]).all() and (st["NG_p"] == g[f"NG_p_{tag}"]).all()
+            ns = g[f"soft_{tag}"].shape[0]
+            assert np.array_equal(soft[:ns], g[f"soft_{tag}"])
+    finally:
+        oracle.set_variant(0)
+    # the fixture tells the two decoders apart: the earlier one always fills its
+    # list (P_notGuess stays 1, so the confidence never reaches 1 - 1e-5)
+    assert (g["curL"] == 3).all()
+    ng0 = np.array([oracle.siso(x)[2].T for x in g["llr"][:200]])
+    assert (ng0 != g["N_guess"][:200]).any()
