@@ -3,7 +3,7 @@
 # decoder: the same five steps, file names and formats; the decoder step runs
 # the drop-in sogrand_amd/cubic_decoder. Files are written to the current
 # directory. Settings: SNR_DB (default 2.0), N_CW codewords (default 1000),
-# SEED (default 1).
+# SEED (default 1), DEC_FLAGS (decoder options, e.g. "--variant legacy").
 set -e
 HERE=$(cd "$(dirname "$0")" && pwd)
 ROOT=$(dirname "$HERE")
@@ -28,7 +28,7 @@ python3 -m sogrand_amd.flow encode "$ORIG_DATA" "$ENCODED_DATA"
 echo "Simulating channel..."
 python3 -m sogrand_amd.flow channel "$ENCODED_DATA" "$CORRUPTED_LLRS" "$SNR_DB" 3 --seed "$SEED"
 echo "Decoding data..."
-"$HERE/cubic_decoder" "$CORRUPTED_LLRS" "$DECODED_DATA"
+"$HERE/cubic_decoder" "$CORRUPTED_LLRS" "$DECODED_DATA" ${DEC_FLAGS:-}
 echo "Comparing results..."
 python3 -m sogrand_amd.flow compare "$ORIG_DATA" "$DECODED_DATA"
 echo "--- Pipeline finished ---"
