@@ -380,6 +380,48 @@ def workload_mixed(args) -> dict:
                        "n_cw": n}, "rows": rows}
 
 
+def workload_variants(args) -> dict:
+    """The reference's three SOGRAND_C decoders on the same kernels: throughput,
+    error rates and work of each, 100,000 codewords per Eb/N0 by default."""
+    import sogrand_amd as sg
+    from sogrand_amd import hip
+
+    n = args.n_cw
+    dec = sg.Decoder(0, backend=sg.BACKEND_PERSISTENT)
+    variants = (("cubic_decoder_final.c", {"variant": sg.VARIANT_FINAL, "epsilon": 1e-40}),
+                ("cubic_decoder_updated.c", {"variant": sg.VARIANT_FINAL,
+                                             "epsilon": 2.220446049250313e-16}),
+                ("cubic_decoder.c", {"variant": sg.VARIANT_LEGACY, "epsilon": 1e-40}))
+    rows = []
+    for snr in (2.0, 3.0):
+        seed = args.seed + int(snr * 10)
+        d_llr = gen_device(dec, seed, n, sg.sigma_for_ebn0(snr))
+        d_bits = hip.DeviceArray((n, 64), np.uint8)
+        d_st = hip.DeviceArray((n,), sg.CW_STATS_DTYPE)
+        info = sg.gen_info_bits(seed, n)
+        for name, kw in variants:
+            dec.set_params(**kw)
+            timed_decode(dec, d_llr, d_bits, d_st)
+            wall, kms = timed_decode(dec, d_llr, d_bits, d_st)
+            bits, st = d_bits.to_host(), d_st.to_host()
+            be, bl = ber_bler(bits, info)
+            rows.append({"decoder": name, "ebn0_db": snr, "cw_s": round(n / wall, 1),
+                         "kernel_ms": round(kms, 2), "ber": be / (512.0 * n),
+                         "ber_ci95": wilson(be, 512 * n), "bler": bl / n,
+                         "bler_ci95": wilson(bl, n),
+                         "mean_half_iters": float(st["half_iters"].mean()),
+                         "converged": float(st["converged"].mean()),
+                         "mean_NG": float(st["NG"].astype(np.float64).mean())})
+            log(f"{snr} dB {name}: {n / wall:.0f} cw/s, BLER {bl / n:.3e}")
+        for x in (d_llr, d_bits, d_st):
+            x.free()
+    return {"metric": "codewords/s of the reference's decoder variants",
+            "unit": "codewords/s", "n_gpus": 1, "dtype": "f64",
+            "data": "synthetic (seeded info bits, (16,8)^3 encoding, BPSK, AWGN)",
+            "config": {"workload": f"{n} codewords at 2.0 and 3.0 dB, persistent kernel, "
+                                   f"L=3, Imax=30", "n_cw": n}, "rows": rows}
+
+
 # ---------------------------------------------------------------------------
 # HBM counter calibration
 
@@ -524,7 +566,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="headline",
-                    choices=["headline", "uneven", "mixed-precision", "serving", "copy"])
+                    choices=["headline", "uneven", "mixed-precision", "serving", "variants",
+                             "copy"])
     ap.add_argument("--n-cw", type=int, default=None)
     ap.add_argument("--ebn0", type=float, default=2.0)
     ap.add_argument("--seed", type=int, default=20260)
@@ -548,7 +591,7 @@ def main() -> int:
         args.n_cw = 100_000
     fn = {"headline": workload_headline, "uneven": workload_uneven,
           "mixed-precision": workload_mixed, "serving": workload_serving,
-          "copy": workload_copy}[args.workload]
+          "variants": workload_variants, "copy": workload_copy}[args.workload]
     print(json.dumps(fn(args)), flush=True)
     return 0
 
