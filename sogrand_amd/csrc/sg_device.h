@@ -568,7 +568,17 @@ __device__ __forceinline__ void search_run(SearchState &ss, LineList &ls, const 
         if (ss.done) continue;
         const uint32_t t = ss.t; // always even here
         if ((t & 7u) == 0) {
-            ss.cur = ss.nxt;
+            // The prefetched chunk is copied out by explicit moves ordered before
+            // the next load. Left to the compiler, the copy becomes a phi copy at
+            // the end of the block, after the load, so the load lands in a
+            // temporary and is moved into the prefetch registers at once, which
+            // waits for it and defeats the prefetch.
+            uint4 c;
+            asm volatile("v_mov_b32 %0, %1" : "=v"(c.x) : "v"(ss.nxt.x) : "memory");
+            asm volatile("v_mov_b32 %0, %1" : "=v"(c.y) : "v"(ss.nxt.y) : "memory");
+            asm volatile("v_mov_b32 %0, %1" : "=v"(c.z) : "v"(ss.nxt.z) : "memory");
+            asm volatile("v_mov_b32 %0, %1" : "=v"(c.w) : "v"(ss.nxt.w) : "memory");
+            ss.cur = c;
             ss.nxt = ss.tp[(t >> 3) + 1]; // the table buffer carries one chunk of padding
         }
         const uint32_t m1 = ss.cur.x & 0xffffu, m2 = ss.cur.x >> 16;

@@ -127,6 +127,17 @@ __device__ __forceinline__ double shfl_d(double v, int src)
     return bits_to_double(((uint64_t)hi << 32) | lo);
 }
 
+// XOR of v over each group of 16 lanes (lanes 16q .. 16q + 15), in every lane
+// of the group: two quad permutations, then the half-row and row mirrors.
+__device__ __forceinline__ uint32_t xor16(uint32_t v)
+{
+    v ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, true);  // quad_perm 1,0,3,2
+    v ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, true);  // quad_perm 2,3,0,1
+    v ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x141, 0xF, 0xF, true); // row_half_mirror
+    v ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x140, 0xF, 0xF, true); // row_mirror
+    return v;
+}
+
 // Four waves per SIMD asked for: the single-precision variant's records fit
 // four workgroups' LDS per CU and its registers fit 128 with a few bytes of
 // spill, which measures faster than three waves; the FP64 kernel's LDS allows
@@ -428,17 +439,22 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))
                 const uint64_t special = __ballot(owner >= 0 && (tie || !fin));
                 const uint64_t negm = __ballot(owner >= 0 && neg);
                 const uint64_t badm = __ballot(owner >= 0 && !fin);
+                // syndrome of the hard decision: XOR of the H columns of the
+                // line's negative positions over its 16 lanes
+                const uint32_t syn16 = xor16(neg ? (uint32_t)hcol_of((unsigned)i) : 0u);
                 // owners: hard decision, PM_HD in natural order, intercept
                 const int myq = lane == o[0] ? 0 : lane == o[1] ? 1 : lane == o[2] ? 2
                               : lane == o[3] ? 3 : -1;
+                // the owner of line q takes the syndrome from the line's lanes
+                const uint32_t syn_owner = (uint32_t)__shfl((int)syn16, 16 * (myq >= 0 ? myq : 0), 64);
                 bool line_special = false, line_bad = false;
                 if (myq >= 0) {
                     line_special = ((special >> (16 * myq)) & 0xffffull) != 0;
                     line_bad = ((badm >> (16 * myq)) & 0xffffull) != 0;
                     hd = (uint32_t)(negm >> (16 * myq)) & 0xffffu;
-                    s0 = syndrome16(hd);
+                    s0 = syn_owner;
                     pm_hd = 0.0;
-#pragma unroll 1
+#pragma unroll
                     for (int j = 0; j < 16; ++j) pm_hd += stage_jn[16 * myq + j]; // getPM_HD (:617)
                     // IC (:619-622)
                     ic = intercept(stage_a[2 * myq], stage_a[2 * myq + 1], P::kLegacy);
