@@ -506,8 +506,7 @@ struct SearchState {
     double pnot;      // P_notGuess (:614, :626, :672, :712)
     bool done;
     const uint4 *tp;  // schedule table of the line's intercept
-    uint4 nxt;        // next 16-byte chunk of 8 masks (prefetched)
-    uint4 cur;        // current chunk, rotated so that cur.x holds the next two masks
+    uint4 nxt;        // 16-byte chunk of the next 8 masks (prefetched)
     int64_t cur_wt;   // weight level of the last pattern (finite Tmax only)
 };
 
@@ -535,18 +534,15 @@ __device__ __forceinline__ void search_init(SearchState &ss, LineList &ls, uint3
     const int tbl = ic < kSchedGenericIc ? ic : kSchedGenericIc;
     ss.tp = reinterpret_cast<const uint4 *>(cfg.sched + (size_t)tbl * kSchedStride);
     ss.nxt = ss.tp[0];
-    ss.cur = uint4{0u, 0u, 0u, 0u};
     ss.t = 0;
     ss.cur_wt = -1;
 }
 
-// Up to max_steps further guesses of the schedule (:652-744), two patterns per
-// iteration so that each lane carries two independent path-metric and exp
-// chains; the guesses are still accounted strictly in schedule order (the
-// second is dropped if the first ends the search). Masks stream 8 per 16-byte
-// load with the next load issued one chunk ahead. The loop runs a step count
-// common to the wave and leaves when no lane is still searching; a finished
-// lane idles inside it, which keeps the wave's control flow to one mask.
+// Two consecutive guesses of the schedule (:652-744): the patterns of `pair`
+// (two 16-bit masks) at schedule positions t and t + 1. Each lane carries two
+// independent path-metric and exp chains; the guesses are still accounted
+// strictly in schedule order (the second is dropped if the first ends the
+// search).
 //
 // In the earlier decoder (P::kLegacy) P_notGuess never changes, so a pattern's
 // metric matters only if it is a codeword or, under a guess budget, the last
@@ -554,111 +550,123 @@ __device__ __forceinline__ void search_init(SearchState &ss, LineList &ls, uint3
 // Without a budget the loop forms syndromes alone and a hit's metric is summed
 // from its mask afterwards, with the same additions.
 template <bool FINITE_T, class P, class LDS>
-__device__ __forceinline__ void search_run(SearchState &ss, LineList &ls, const LDS &S, uint32_t hd,
-                                           uint32_t s0, double pm_hd, int32_t ic,
-                                           const DecodeCfg &cfg, uint32_t max_steps)
+__device__ __forceinline__ void search_pair(SearchState &ss, LineList &ls, const LDS &S,
+                                            uint32_t s0, double pm_hd, int32_t ic,
+                                            const DecodeCfg &cfg, uint32_t pair, uint32_t t)
 {
     using Rec = typename P::Rec;
     static_assert(sizeof(Rec) == sizeof(typename LDS::Rec), "record type");
     constexpr bool kPmLoop = !(P::kLegacy && !FINITE_T);
+    const uint32_t m1 = pair & 0xffffu, m2 = pair >> 16;
+    if (m1 == 0) { // all 65535 patterns tested (wt > wt_max)
+        ss.done = true;
+        return;
+    }
+    bool take2 = m2 != 0;
+    if (FINITE_T) { // loop guard once per weight level (:652)
+        const int64_t wt1 = (int64_t)ic * __builtin_popcount(m1) + position_sum(m1);
+        const int64_t wt2 = (int64_t)ic * __builtin_popcount(m2) + position_sum(m2);
+        if (wt1 != ss.cur_wt) {
+            if (!((uint64_t)ss.T < cfg.tmax)) {
+                ss.done = true;
+                return;
+            }
+            ss.cur_wt = wt1;
+        }
+        if (take2 && wt2 != wt1 && !((uint64_t)ss.T + 1u < cfg.tmax)) take2 = false;
+        else if (take2) ss.cur_wt = wt2;
+    }
+    // path metrics (ascending sorted position, as getPM adds them) and
+    // syndromes, two set bits of each pattern per iteration; an exhausted mask
+    // gives position -1, the null record (delta +0.0, H column 0), which leaves
+    // both sums unchanged, so no lane needs a select or a branch
     const Rec pm_base = (Rec)pm_hd;
+    Rec pm1 = pm_base, pm2 = pm_base;
+    uint32_t syn1 = s0, syn2 = s0;
+    uint32_t r1 = m1, r2 = take2 ? m2 : 0u;
+    do {
+        const uint32_t q1 = r1 & (r1 - 1u), q2 = r2 & (r2 - 1u);
+        const int b1 = lowest_bit(r1), c1 = lowest_bit(q1);
+        const int b2 = lowest_bit(r2), c2 = lowest_bit(q2);
+        syn1 ^= (uint32_t)S.h(b1) ^ (uint32_t)S.h(c1);
+        syn2 ^= (uint32_t)S.h(b2) ^ (uint32_t)S.h(c2);
+        if constexpr (kPmLoop) {
+            const Rec d1 = S.d(b1), e1 = S.d(c1), d2 = S.d(b2), e2 = S.d(c2);
+            pm1 = (pm1 + d1) + e1;
+            pm2 = (pm2 + d2) + e2;
+        }
+        r1 = q1 & (q1 - 1u);
+        r2 = q2 & (q2 - 1u);
+    } while (r1 | r2);
+    const bool hit1 = (syn1 & 0xffu) == 0, hit2 = (syn2 & 0xffu) == 0;
+    double e1 = 0.0, e2 = 0.0;
+    if constexpr (P::kF32) {
+        // single-precision weights for the found-mass chain; a hit's list
+        // weight is taken in double from its (single-precision) metric, in the
+        // (rare) hit branch below
+        e1 = exp_neg_f32((float)pm1);
+        e2 = exp_neg_f32((float)pm2);
+    } else if constexpr (kPmLoop) {
+        e1 = exp_neg((double)pm1, cfg.exptab);
+        e2 = exp_neg((double)pm2, cfg.exptab);
+    }
+    ss.T += 1;
+    ss.t = t + 1;
+    if (P::kF32 && hit1) e1 = exp_neg((double)pm1, cfg.exptab);
+    if (!kPmLoop && hit1) e1 = exp_neg(pattern_pm<P>(S, m1, pm_hd), cfg.exptab);
+    if constexpr (!P::kLegacy)
+        ss.pnot -= e1;
+    else if (kPmLoop && !hit1)
+        ls.pnl = e1; // cubic_decoder.c:638
+    bool stop = hit1 && list_hit(ls, e1, t + 1u, ss.pnot, cfg);
+    // end of the schedule, or the next weight level is over budget
+    if (!stop && !take2) stop = true;
+    if (!stop) {
+        ss.T += 1;
+        ss.t = t + 2;
+        if (P::kF32 && hit2) e2 = exp_neg((double)pm2, cfg.exptab);
+        if (!kPmLoop && hit2) e2 = exp_neg(pattern_pm<P>(S, m2, pm_hd), cfg.exptab);
+        if constexpr (!P::kLegacy)
+            ss.pnot -= e2;
+        else if (kPmLoop && !hit2)
+            ls.pnl = e2;
+        stop = hit2 && list_hit(ls, e2, t + 2u, ss.pnot, cfg);
+    }
+    ss.done = stop;
+}
+
+// Further guesses of the schedule, one 16-byte chunk of 8 masks (four pairs)
+// per iteration, up to about max_steps guesses. A search starts at a chunk
+// boundary and a lane that stops mid-chunk is done, so every searching lane
+// enters an iteration at a chunk boundary; the chunk was loaded one iteration
+// ahead. The loop runs an iteration count common to the wave and leaves when
+// no lane is still searching; a finished lane idles inside it, which keeps the
+// wave's control flow to one mask.
+template <bool FINITE_T, class P, class LDS>
+__device__ __forceinline__ void search_run(SearchState &ss, LineList &ls, const LDS &S, uint32_t hd,
+                                           uint32_t s0, double pm_hd, int32_t ic,
+                                           const DecodeCfg &cfg, uint32_t max_steps)
+{
 #pragma unroll 1
-    for (uint32_t step = 0; step < max_steps; step += 2) {
+    for (uint32_t step = 0; step < max_steps; step += 8) {
         if (__ballot(!ss.done) == 0ull) break;
         if (ss.done) continue;
-        const uint32_t t = ss.t; // always even here
-        if ((t & 7u) == 0) {
-            // The prefetched chunk is copied out by explicit moves ordered before
-            // the next load. Left to the compiler, the copy becomes a phi copy at
-            // the end of the block, after the load, so the load lands in a
-            // temporary and is moved into the prefetch registers at once, which
-            // waits for it and defeats the prefetch.
-            uint4 c;
-            asm volatile("v_mov_b32 %0, %1" : "=v"(c.x) : "v"(ss.nxt.x) : "memory");
-            asm volatile("v_mov_b32 %0, %1" : "=v"(c.y) : "v"(ss.nxt.y) : "memory");
-            asm volatile("v_mov_b32 %0, %1" : "=v"(c.z) : "v"(ss.nxt.z) : "memory");
-            asm volatile("v_mov_b32 %0, %1" : "=v"(c.w) : "v"(ss.nxt.w) : "memory");
-            ss.cur = c;
-            ss.nxt = ss.tp[(t >> 3) + 1]; // the table buffer carries one chunk of padding
-        }
-        const uint32_t m1 = ss.cur.x & 0xffffu, m2 = ss.cur.x >> 16;
-        ss.cur.x = ss.cur.y;
-        ss.cur.y = ss.cur.z;
-        ss.cur.z = ss.cur.w;
-        if (m1 == 0) { // all 65535 patterns tested (wt > wt_max)
-            ss.done = true;
-            continue;
-        }
-        bool take2 = m2 != 0;
-        if (FINITE_T) { // loop guard once per weight level (:652)
-            const int64_t wt1 = (int64_t)ic * __builtin_popcount(m1) + position_sum(m1);
-            const int64_t wt2 = (int64_t)ic * __builtin_popcount(m2) + position_sum(m2);
-            if (wt1 != ss.cur_wt) {
-                if (!((uint64_t)ss.T < cfg.tmax)) {
-                    ss.done = true;
-                    continue;
-                }
-                ss.cur_wt = wt1;
-            }
-            if (take2 && wt2 != wt1 && !((uint64_t)ss.T + 1u < cfg.tmax)) take2 = false;
-            else if (take2) ss.cur_wt = wt2;
-        }
-        // path metrics (ascending sorted position, as getPM adds them) and
-        // syndromes, two set bits of each pattern per iteration; an exhausted
-        // mask gives position -1, the null record (delta +0.0, H column 0),
-        // which leaves both sums unchanged, so no lane needs a select or a branch
-        Rec pm1 = pm_base, pm2 = pm_base;
-        uint32_t syn1 = s0, syn2 = s0;
-        uint32_t r1 = m1, r2 = take2 ? m2 : 0u;
-        do {
-            const uint32_t q1 = r1 & (r1 - 1u), q2 = r2 & (r2 - 1u);
-            const int b1 = lowest_bit(r1), c1 = lowest_bit(q1);
-            const int b2 = lowest_bit(r2), c2 = lowest_bit(q2);
-            syn1 ^= (uint32_t)S.h(b1) ^ (uint32_t)S.h(c1);
-            syn2 ^= (uint32_t)S.h(b2) ^ (uint32_t)S.h(c2);
-            if constexpr (kPmLoop) {
-                const Rec d1 = S.d(b1), e1 = S.d(c1), d2 = S.d(b2), e2 = S.d(c2);
-                pm1 = (pm1 + d1) + e1;
-                pm2 = (pm2 + d2) + e2;
-            }
-            r1 = q1 & (q1 - 1u);
-            r2 = q2 & (q2 - 1u);
-        } while (r1 | r2);
-        const bool hit1 = (syn1 & 0xffu) == 0, hit2 = (syn2 & 0xffu) == 0;
-        double e1 = 0.0, e2 = 0.0;
-        if constexpr (P::kF32) {
-            // single-precision weights for the found-mass chain; a hit's list
-            // weight is taken in double from its (single-precision) metric,
-            // in the (rare) hit branch below
-            e1 = exp_neg_f32((float)pm1);
-            e2 = exp_neg_f32((float)pm2);
-        } else if constexpr (kPmLoop) {
-            e1 = exp_neg((double)pm1, cfg.exptab);
-            e2 = exp_neg((double)pm2, cfg.exptab);
-        }
-        ss.T += 1;
-        ss.t = t + 1;
-        if (P::kF32 && hit1) e1 = exp_neg((double)pm1, cfg.exptab);
-        if (!kPmLoop && hit1) e1 = exp_neg(pattern_pm<P>(S, m1, pm_hd), cfg.exptab);
-        if constexpr (!P::kLegacy)
-            ss.pnot -= e1;
-        else if (kPmLoop && !hit1)
-            ls.pnl = e1; // cubic_decoder.c:638
-        bool stop = hit1 && list_hit(ls, e1, t + 1u, ss.pnot, cfg);
-        // end of the schedule, or the next weight level is over budget
-        if (!stop && !take2) stop = true;
-        if (!stop) {
-            ss.T += 1;
-            ss.t = t + 2;
-            if (P::kF32 && hit2) e2 = exp_neg((double)pm2, cfg.exptab);
-            if (!kPmLoop && hit2) e2 = exp_neg(pattern_pm<P>(S, m2, pm_hd), cfg.exptab);
-            if constexpr (!P::kLegacy)
-                ss.pnot -= e2;
-            else if (kPmLoop && !hit2)
-                ls.pnl = e2;
-            stop = hit2 && list_hit(ls, e2, t + 2u, ss.pnot, cfg);
-        }
-        ss.done = stop;
+        const uint32_t t = ss.t; // a multiple of 8 here
+        // The prefetched chunk is copied out by explicit moves ordered before
+        // the next load. Left to the compiler, the copy becomes a phi copy at
+        // the end of the block, after the load, so the load lands in a
+        // temporary and is moved into the prefetch registers at once, which
+        // waits for it and defeats the prefetch.
+        uint4 c;
+        asm volatile("v_mov_b32 %0, %1" : "=v"(c.x) : "v"(ss.nxt.x) : "memory");
+        asm volatile("v_mov_b32 %0, %1" : "=v"(c.y) : "v"(ss.nxt.y) : "memory");
+        asm volatile("v_mov_b32 %0, %1" : "=v"(c.z) : "v"(ss.nxt.z) : "memory");
+        asm volatile("v_mov_b32 %0, %1" : "=v"(c.w) : "v"(ss.nxt.w) : "memory");
+        ss.nxt = ss.tp[(t >> 3) + 1]; // the table buffer carries one chunk of padding
+        search_pair<FINITE_T, P>(ss, ls, S, s0, pm_hd, ic, cfg, c.x, t);
+        if (!ss.done) search_pair<FINITE_T, P>(ss, ls, S, s0, pm_hd, ic, cfg, c.y, t + 2u);
+        if (!ss.done) search_pair<FINITE_T, P>(ss, ls, S, s0, pm_hd, ic, cfg, c.z, t + 4u);
+        if (!ss.done) search_pair<FINITE_T, P>(ss, ls, S, s0, pm_hd, ic, cfg, c.w, t + 6u);
     }
 }
 
