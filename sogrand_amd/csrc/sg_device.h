@@ -60,8 +60,10 @@ __device__ __forceinline__ void load_exp_table(ExpEntry *lds, int tid)
 __device__ __forceinline__ double exp_neg(double pm, const ExpEntry *tab)
 {
     const bool under = !(pm < 745.5);
-    const double x = -fmin(pm, 745.5);
-    const double kd = __builtin_rint(x * kExpInvLn2N);
+    const double x = -pm;
+    // the bound only keeps k in int range for pm >= 745.5 (and NaN), whose
+    // result is replaced by 0 below; below 745.5 it is never active
+    const double kd = __builtin_fmax(__builtin_rint(x * kExpInvLn2N), -34432.0);
     const int k = (int)kd;
     const ExpEntry &t = tab[k & 31];
     double r = __builtin_fma(-kd, kExpLn2NHi, x);
@@ -557,11 +559,10 @@ __device__ __forceinline__ void search_pair(SearchState &ss, LineList &ls, const
     using Rec = typename P::Rec;
     static_assert(sizeof(Rec) == sizeof(typename LDS::Rec), "record type");
     constexpr bool kPmLoop = !(P::kLegacy && !FINITE_T);
+    // Masks 0 .. 65534 of a schedule are nonzero and mask 65535 is the zero
+    // pad, so m1 is never 0: the pair (65534, 65535) ends the search through
+    // take2 = false (all patterns tested, wt > wt_max).
     const uint32_t m1 = pair & 0xffffu, m2 = pair >> 16;
-    if (m1 == 0) { // all 65535 patterns tested (wt > wt_max)
-        ss.done = true;
-        return;
-    }
     bool take2 = m2 != 0;
     if (FINITE_T) { // loop guard once per weight level (:652)
         const int64_t wt1 = (int64_t)ic * __builtin_popcount(m1) + position_sum(m1);
@@ -580,11 +581,26 @@ __device__ __forceinline__ void search_pair(SearchState &ss, LineList &ls, const
     // syndromes, two set bits of each pattern per iteration; an exhausted mask
     // gives position -1, the null record (delta +0.0, H column 0), which leaves
     // both sums unchanged, so no lane needs a select or a branch
+    // (the first round reads the hard decision's metric and syndrome directly)
     const Rec pm_base = (Rec)pm_hd;
-    Rec pm1 = pm_base, pm2 = pm_base;
-    uint32_t syn1 = s0, syn2 = s0;
+    Rec pm1, pm2;
+    uint32_t syn1, syn2;
     uint32_t r1 = m1, r2 = take2 ? m2 : 0u;
-    do {
+    {
+        const uint32_t q1 = r1 & (r1 - 1u), q2 = r2 & (r2 - 1u);
+        const int b1 = lowest_bit(r1), c1 = lowest_bit(q1);
+        const int b2 = lowest_bit(r2), c2 = lowest_bit(q2);
+        syn1 = s0 ^ (uint32_t)S.h(b1) ^ (uint32_t)S.h(c1);
+        syn2 = s0 ^ (uint32_t)S.h(b2) ^ (uint32_t)S.h(c2);
+        if constexpr (kPmLoop) {
+            const Rec d1 = S.d(b1), e1 = S.d(c1), d2 = S.d(b2), e2 = S.d(c2);
+            pm1 = (pm_base + d1) + e1;
+            pm2 = (pm_base + d2) + e2;
+        }
+        r1 = q1 & (q1 - 1u);
+        r2 = q2 & (q2 - 1u);
+    }
+    while (r1 | r2) {
         const uint32_t q1 = r1 & (r1 - 1u), q2 = r2 & (r2 - 1u);
         const int b1 = lowest_bit(r1), c1 = lowest_bit(q1);
         const int b2 = lowest_bit(r2), c2 = lowest_bit(q2);
@@ -597,7 +613,7 @@ __device__ __forceinline__ void search_pair(SearchState &ss, LineList &ls, const
         }
         r1 = q1 & (q1 - 1u);
         r2 = q2 & (q2 - 1u);
-    } while (r1 | r2);
+    }
     const bool hit1 = (syn1 & 0xffu) == 0, hit2 = (syn2 & 0xffu) == 0;
     double e1 = 0.0, e2 = 0.0;
     if constexpr (P::kF32) {
