@@ -177,7 +177,7 @@ typedef struct sogrand_server sogrand_server;
 
 typedef struct sogrand_server_cfg {
     size_t hbm_budget;       /* device bytes for all batch slots (default 4 GiB)          */
-    uint32_t max_inflight;   /* batch slots = concurrent launches, 1..16 (default 16; they
+    uint32_t max_inflight;   /* batch slots = concurrent launches, 1..32 (default 16; they
                                 run concurrently up to the HIP runtime's hardware queues,
                                 GPU_MAX_HW_QUEUES)                                          */
     uint32_t max_batch_cw;   /* codewords per batch (0 = what the budget allows, <= 65536) */

@@ -302,7 +302,7 @@ int sogrand_server_create(sogrand_server **out, sogrand_ctx *ctx, const sogrand_
         c = *cfg;
     else
         sogrand_server_cfg_default(&c);
-    if (c.max_inflight < 1 || c.max_inflight > 16) return SOGRAND_EINVAL;
+    if (c.max_inflight < 1 || c.max_inflight > 32) return SOGRAND_EINVAL;
     sogrand_server *s = new (std::nothrow) sogrand_server;
     if (!s) return SOGRAND_ENOMEM;
     s->ctx = ctx;
