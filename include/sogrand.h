@@ -122,8 +122,10 @@ int sogrand_decode_cubic(sogrand_ctx *ctx, const double *llr, size_t n_cw, uint8
 int sogrand_bitso_lines(sogrand_ctx *ctx, const double *llr, size_t n_lines, double *L_APP,
                         double *L_E, uint32_t *N_guess, void *hip_stream);
 
-/* Time (ms, HIP events on the launch stream) of the decode kernels of the last
- * sogrand_decode_cubic call on this context, and how many launches it made. */
+/* Time (ms, HIP events) of the decode kernels of the last sogrand_decode_cubic
+ * call on this context, from the first kernel's start to the last kernel's end
+ * (with host buffers, chunks are decoded on two alternating streams and their
+ * launches overlap), and how many launches it made. */
 int sogrand_last_kernel_ms(sogrand_ctx *ctx, double *ms, uint32_t *launches);
 
 /* With use_graph: how many times the context's current captured graph has been

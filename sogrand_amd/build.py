@@ -65,7 +65,8 @@ def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", *objs, "-o", tmp])
     os.replace(tmp, LIB)
-    cli_cmd = [HIPCC, *COMMON, os.path.join(CSRC, "cubic_decoder_main.cpp"), "-o", CLI + ".tmp",
+    cli_cmd = [HIPCC, *COMMON, "-pthread", os.path.join(CSRC, "cubic_decoder_main.cpp"), "-o",
+               CLI + ".tmp",
                "-L" + PKG, "-lsogrand", "-Wl,-rpath,$ORIGIN"]
     _run(cli_cmd)
     os.replace(CLI + ".tmp", CLI)

@@ -22,6 +22,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "sogrand.h"
@@ -117,25 +118,41 @@ int main(int argc, char *argv[])
     if (p.variant == SOGRAND_VARIANT_LEGACY)
         std::printf("Decoding %s to %s using cubic code (n=%d, k=%d)...\n", in_path, out_path,
                     SOGRAND_N, SOGRAND_K);
+    // chunks of the file alternate between two buffers: the next one is read by
+    // a second thread while the current one decodes
     const size_t chunk = 65536;
-    std::vector<double> llr;
+    std::vector<double> llr[2];
+    bool read_ok[2] = {false, false};
+    auto read_chunk = [&](int b, size_t n) {
+        llr[b].resize(n * SOGRAND_CUBE);
+        read_ok[b] = std::fread(llr[b].data(), sizeof(double), n * SOGRAND_CUBE, fin) ==
+                     n * SOGRAND_CUBE;
+    };
     std::vector<uint8_t> bits;
     std::vector<sogrand_cw_stats> st;
     std::vector<double> soft;
     uint64_t total_ng = 0, total_ngp = 0, total_half = 0;
     double gpu_ms = 0.0;
-    for (size_t off = 0; off < total; off += chunk) {
+    read_chunk(0, total < chunk ? total : chunk);
+    int cur = 0;
+    for (size_t off = 0; off < total; off += chunk, cur ^= 1) {
         const size_t n = (total - off) < chunk ? (total - off) : chunk;
-        llr.resize(n * SOGRAND_CUBE);
+        std::thread reader;
+        if (off + chunk < total) {
+            const size_t left = total - off - chunk;
+            reader = std::thread(read_chunk, cur ^ 1, left < chunk ? left : chunk);
+        }
         bits.resize(n * SOGRAND_INFO_BYTES);
         st.resize(n);
         if (fsoft) soft.resize(n * SOGRAND_CUBE);
-        if (std::fread(llr.data(), sizeof(double), n * SOGRAND_CUBE, fin) != n * SOGRAND_CUBE) {
+        if (!read_ok[cur]) {
+            if (reader.joinable()) reader.join();
             std::fprintf(stderr, "Error: Failed to read entire file\n");
             return 1;
         }
-        rc = sogrand_decode_cubic(ctx, llr.data(), n, bits.data(), fsoft ? soft.data() : nullptr,
-                                  st.data(), nullptr);
+        rc = sogrand_decode_cubic(ctx, llr[cur].data(), n, bits.data(),
+                                  fsoft ? soft.data() : nullptr, st.data(), nullptr);
+        if (reader.joinable()) reader.join();
         if (rc) {
             std::fprintf(stderr, "Error: %s\n", sogrand_strerror(rc));
             return 1;

@@ -249,6 +249,19 @@ void sogrand_ctx_destroy(sogrand_ctx *ctx)
     ctx->queue.release();
     ctx->status.release();
     ctx->nguess_stage.release();
+    for (HostStage &h : ctx->hs) {
+        if (h.st) (void)hipStreamSynchronize(h.st);
+        h.llr.release();
+        h.soft.release();
+        h.le.release();
+        h.info.release();
+        h.stats.release();
+        h.queue.release();
+        h.status.release();
+        if (h.ev0) (void)hipEventDestroy(h.ev0);
+        if (h.ev1) (void)hipEventDestroy(h.ev1);
+        if (h.st) (void)hipStreamDestroy(h.st);
+    }
     if (ctx->d_sched) (void)hipFree(ctx->d_sched);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
@@ -336,6 +349,141 @@ static int decode_device_chunk(sogrand_ctx *ctx, const double *d_llr, size_t n, 
     return SOGRAND_OK;
 }
 
+// Results of the chunk in flight on stage h to the caller's host buffers (or,
+// for device-resident outputs, nothing to copy), after its stream has finished.
+static int finish_stage(sogrand_ctx *ctx, HostStage &h, uint8_t *info_bits, double *soft_out,
+                        sogrand_cw_stats *stats, bool dev_info, bool dev_soft, bool dev_stats)
+{
+    if (!h.busy) return SOGRAND_OK;
+    h.busy = false;
+    if (hipStreamSynchronize(h.st) != hipSuccess) return SOGRAND_EHIP;
+    const size_t off = h.off, n = h.n;
+    std::vector<int32_t> status(n);
+    if (hipMemcpy(status.data(), h.status.p, n * sizeof(int32_t), hipMemcpyDeviceToHost) !=
+        hipSuccess)
+        return SOGRAND_EHIP;
+    for (size_t i = 0; i < n; ++i)
+        if (status[i]) return SOGRAND_ENONFINITE;
+    if (!dev_info && hipMemcpy(info_bits + off * sg::kInfoBytes, h.info.p, n * sg::kInfoBytes,
+                               hipMemcpyDeviceToHost) != hipSuccess)
+        return SOGRAND_EHIP;
+    if (soft_out && !dev_soft &&
+        hipMemcpy(soft_out + off * sg::kCube, h.soft.p, n * sg::kCube * sizeof(double),
+                  hipMemcpyDeviceToHost) != hipSuccess)
+        return SOGRAND_EHIP;
+    if (stats && !dev_stats &&
+        hipMemcpy(stats + off, h.stats.p, n * sizeof(sogrand_cw_stats), hipMemcpyDeviceToHost) !=
+            hipSuccess)
+        return SOGRAND_EHIP;
+    (void)ctx;
+    return SOGRAND_OK;
+}
+
+// Decode with some buffers in host memory: balanced chunks of at most 32,768
+// codewords alternate between two stages, so that each chunk's input copy and
+// launch overlap the previous chunk's kernel (and its tail), and each stage's
+// results are copied out when the stage is next needed.
+static int decode_host_pipelined(sogrand_ctx *ctx, const double *llr, size_t n_cw,
+                                 uint8_t *info_bits, double *soft_out, sogrand_cw_stats *stats,
+                                 bool dev_llr, bool dev_info, bool dev_soft, bool dev_stats,
+                                 float *ms)
+{
+    const sogrand_params &p = ctx->p;
+    const size_t max_chunk = 32768;
+    const size_t n_chunks = (n_cw + max_chunk - 1) / max_chunk;
+    const size_t chunk = (n_cw + n_chunks - 1) / n_chunks;
+    for (HostStage &h : ctx->hs) {
+        if (!h.st && hipStreamCreateWithFlags(&h.st, hipStreamNonBlocking) != hipSuccess) {
+            h.st = nullptr;
+            return SOGRAND_EHIP;
+        }
+        if (!h.ev0 && hipEventCreate(&h.ev0) != hipSuccess) return SOGRAND_EHIP;
+        if (!h.ev1 && hipEventCreate(&h.ev1) != hipSuccess) return SOGRAND_EHIP;
+        h.busy = false; // every call finishes both stages before it returns
+    }
+    int rc = SOGRAND_OK;
+    size_t k = 0;
+    for (size_t off = 0; off < n_cw; off += chunk, ++k) {
+        HostStage &h = ctx->hs[k & 1];
+        rc = finish_stage(ctx, h, info_bits, soft_out, stats, dev_info, dev_soft, dev_stats);
+        if (rc != SOGRAND_OK) break;
+        const size_t n = (n_cw - off) < chunk ? (n_cw - off) : chunk;
+        const double *d_llr = llr + off * sg::kCube;
+        if (!dev_llr) {
+            if (h.llr.reserve(n * sg::kCube) != hipSuccess) { rc = SOGRAND_ENOMEM; break; }
+            if (hipMemcpyAsync(h.llr.p, d_llr, n * sg::kCube * sizeof(double),
+                               hipMemcpyHostToDevice, h.st) != hipSuccess)
+                { rc = SOGRAND_EHIP; break; }
+            d_llr = h.llr.p;
+        }
+        uint8_t *d_info = info_bits + off * sg::kInfoBytes;
+        if (!dev_info) {
+            if (h.info.reserve(n * sg::kInfoBytes) != hipSuccess) { rc = SOGRAND_ENOMEM; break; }
+            d_info = h.info.p;
+        }
+        double *d_soft = nullptr;
+        if (soft_out) {
+            d_soft = soft_out + off * sg::kCube;
+            if (!dev_soft) {
+                if (h.soft.reserve(n * sg::kCube) != hipSuccess) { rc = SOGRAND_ENOMEM; break; }
+                d_soft = h.soft.p;
+            }
+        }
+        uint32_t *d_stats = nullptr;
+        if (stats) {
+            d_stats = (uint32_t *)(stats + off);
+            if (!dev_stats) {
+                if (h.stats.reserve(n * 6) != hipSuccess) { rc = SOGRAND_ENOMEM; break; }
+                d_stats = h.stats.p;
+            }
+        }
+        if (h.status.reserve(n) != hipSuccess) { rc = SOGRAND_ENOMEM; break; }
+        sg::CubeJob job{};
+        job.llr = d_llr;
+        job.info = d_info;
+        job.soft = d_soft;
+        job.stats = d_stats;
+        job.status = h.status.p;
+        job.n_cw = n;
+        sg::fill_job(job, ctx, p);
+        const sg::LaunchPlan lp = sg::plan_launch(ctx, p, n);
+        if (h.le.reserve(lp.le_slots * sg::kCube) != hipSuccess) { rc = SOGRAND_ENOMEM; break; }
+        if (h.queue.reserve(4) != hipSuccess) { rc = SOGRAND_ENOMEM; break; }
+        job.le_scratch = h.le.p;
+        job.queue = h.queue.p;
+        if (sg::launch_cubes(p, lp, job, h.st, k == 0 ? ctx->ev0 : h.ev0) != hipSuccess) {
+            rc = SOGRAND_EHIP;
+            break;
+        }
+        if (hipEventRecord(h.ev1, h.st) != hipSuccess) { rc = SOGRAND_EHIP; break; }
+        h.off = off;
+        h.n = n;
+        h.busy = true;
+        ctx->last_launches += 1;
+    }
+    // finish both stages whatever happened, the later chunk last
+    for (size_t j = 0; j < 2; ++j) {
+        HostStage &h = ctx->hs[(k + j) & 1];
+        const int r = finish_stage(ctx, h, info_bits, soft_out, stats, dev_info, dev_soft,
+                                   dev_stats);
+        if (rc == SOGRAND_OK) rc = r;
+    }
+    // an error path can leave a copy enqueued on a stage that never launched
+    for (HostStage &h : ctx->hs) (void)hipStreamSynchronize(h.st);
+    // kernel time: from the first kernel's start to the last kernel's end (the
+    // launches overlap, so their durations do not add up to it)
+    if (rc == SOGRAND_OK && k > 0) {
+        float span = 0.f;
+        for (size_t j = 0; j < 2 && j < k; ++j) {
+            float t = 0.f;
+            if (hipEventElapsedTime(&t, ctx->ev0, ctx->hs[j].ev1) == hipSuccess && t > span)
+                span = t;
+        }
+        *ms += span;
+    }
+    return rc;
+}
+
 int sogrand_decode_cubic(sogrand_ctx *ctx, const double *llr, size_t n_cw, uint8_t *info_bits,
                          double *soft_out, sogrand_cw_stats *stats, void *hip_stream)
 {
@@ -358,7 +506,11 @@ int sogrand_decode_cubic(sogrand_ctx *ctx, const double *llr, size_t n_cw, uint8
         rc = decode_device_chunk(ctx, llr, n_cw, info_bits, soft_out, (uint32_t *)stats, st, &ms,
                                  ctx->p.use_graph != 0);
         ctx->last_launches = 1;
+    } else if (!hip_stream) {
+        rc = decode_host_pipelined(ctx, llr, n_cw, info_bits, soft_out, stats, dev_llr, dev_info,
+                                   dev_soft, dev_stats, &ms);
     } else {
+        // on the caller's stream: one chunk at a time, in order
         const size_t chunk = 65536;
         for (size_t off = 0; off < n_cw && rc == SOGRAND_OK; off += chunk) {
             const size_t n = (n_cw - off) < chunk ? (n_cw - off) : chunk;

@@ -54,6 +54,20 @@ struct GraphKey {
     sogrand_params p;
 };
 
+// One of the two alternating stages of a decode of host buffers: its own
+// stream, staging buffers, L_E scratch, work counter, status and timing events,
+// so that one chunk's input copy and kernel overlap the previous chunk's kernel.
+struct HostStage {
+    hipStream_t st = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    sg::DevBuf<double> llr, soft, le;
+    sg::DevBuf<uint8_t> info;
+    sg::DevBuf<uint32_t> stats, queue;
+    sg::DevBuf<int32_t> status;
+    size_t off = 0, n = 0; // the chunk in flight
+    bool busy = false;
+};
+
 struct sogrand_ctx {
     int device = 0;
     sogrand_params p{};
@@ -71,6 +85,7 @@ struct sogrand_ctx {
     hipGraphExec_t graph = nullptr;
     GraphKey graph_key{};
     uint32_t graph_replays = 0; // replays of the cached graph (tests)
+    HostStage hs[2];
     std::mutex mu;
 };
 
