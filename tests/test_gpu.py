@@ -198,6 +198,35 @@ def test_device_resident_buffers(decoder):
     assert (h.bits == sg.gen_info_bits(5, n)).mean() > 0.999
 
 
+def test_host_buffers_in_overlapped_chunks(decoder):
+    # more than 65,536 codewords from host memory: three chunks alternate
+    # between the two staging streams (one stage is reused), with soft outputs
+    # and statistics; everything equals a single device-resident decode, and a
+    # non-finite value in the last chunk is still reported
+    sg = _sg()
+    from sogrand_amd import hip
+
+    n = 66000
+    decoder.set_params(backend=1)
+    llr = sg.gen_llr(17, n, ebn0_db=3.0)
+    _say("host chunks:", n, "codewords")
+    h = decoder.decode(llr, soft=True)
+    assert decoder.last_kernel_ms()[1] == 3
+    d_llr = hip.DeviceArray.from_host(llr)
+    d_bits = hip.DeviceArray((n, 64), np.uint8)
+    d_st = hip.DeviceArray((n,), sg.CW_STATS_DTYPE)
+    d_soft = hip.DeviceArray((n, 4096))
+    decoder.decode(d_llr, out_bits=d_bits, out_stats=d_st, out_soft=d_soft)
+    assert (d_bits.to_host() == h.bits).all() and (d_st.to_host() == h.stats).all()
+    assert (d_soft.to_host() == h.soft).all()
+    d_llr.free()
+    d_soft.free()
+    llr[n - 5, 9] = np.nan
+    with pytest.raises(sg.SograndError) as e:
+        decoder.decode(llr)
+    assert e.value.code == sg.ENONFINITE
+
+
 @pytest.mark.parametrize("bad", [np.nan, np.inf, -np.inf])
 def test_nonfinite_input_is_an_error(decoder, bad):
     sg = _sg()
