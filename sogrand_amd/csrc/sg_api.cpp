@@ -73,7 +73,7 @@ LaunchPlan plan_launch(const sogrand_ctx *ctx, const sogrand_params &p, size_t n
         if ((size_t)blocks > max_blocks) blocks = (int)max_blocks;
         lp.blocks = blocks;
         lp.slots = slots;
-        lp.le_slots = (size_t)blocks * wpb * slots;
+        lp.le_slots = (size_t)blocks * wpb * slots * kPersistentSlotScratch;
     } else {
         lp.blocks = (int)plain_scratch_slots(n);
         lp.slots = 1;
@@ -82,7 +82,12 @@ LaunchPlan plan_launch(const sogrand_ctx *ctx, const sogrand_params &p, size_t n
     return lp;
 }
 
-size_t max_le_slots(size_t n) { return n + 2 * (size_t)kPersistentWaves * 4; }
+size_t max_le_slots(size_t n)
+{
+    // persistent: at most ceil(n / (waves x slots)) + 1 workgroups, i.e. at most
+    // n + 2 x waves x slots cube slots (slots <= 4), each with its scratch cubes
+    return kPersistentSlotScratch * (n + 2 * (size_t)kPersistentWaves * 4);
+}
 
 void fill_job(CubeJob &job, const sogrand_ctx *ctx, const sogrand_params &p)
 {

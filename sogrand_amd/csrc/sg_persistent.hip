@@ -35,6 +35,13 @@ struct CwStatsOut {
     uint64_t NG_p;
 };
 
+// Device scratch per cube slot, in cubes of 4096 doubles: L_E in natural order,
+// then the inputs x = L_ch + alpha L_E of the current phase's lines, line-major
+// (16 line + i), written by the line set-up and read back by the soft output of
+// the same line, which then reads 16 consecutive doubles instead of two strided
+// sets (the line's own L_E positions are not written in between).
+constexpr int kSlotScratch = kPersistentSlotScratch;
+
 __device__ __forceinline__ void axis_geom(int ax, int line, int &base, int &stride)
 {
     const int x = line & 15, y = line >> 4;
@@ -49,21 +56,6 @@ __device__ __forceinline__ void axis_geom(int ax, int line, int &base, int &stri
         stride = 256;
     }
 }
-
-struct PSrc {
-    const double *lch;
-    const double *le;
-    double alpha;
-    int base, stride;
-    bool first;
-    __device__ double operator()(int i) const
-    {
-        const int p = base + i * stride;
-        const double e = first ? 0.0 : le[p];
-        const double t = alpha * e;
-        return lch[p] + t;
-    }
-};
 
 // Early-termination test of one 16x16 bit plane streamed word by word: word j
 // (bit q = column q) is checked as a component codeword itself when `rows` is
@@ -199,7 +191,7 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))
         const uint32_t ax = g & 0xffu, it = g >> 8;
         axis_geom((int)ax, line, base, stride);
         lch = job.llr + (size_t)c * kCube;
-        le = job.le_scratch + (wave_id * S + (size_t)s) * kCube;
+        le = job.le_scratch + (wave_id * S + (size_t)s) * (kSlotScratch * kCube);
         soft = job.soft ? job.soft + (size_t)c * kCube : nullptr;
         first = (it == 1 && ax == 0);
     };
@@ -308,6 +300,10 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))
             // 3a. soft output of finished lines (SOGRAND_bitSO :536-575): lane
             // 16q + i computes position i of the q-th finished line
             uint64_t D = __ballot(state == kDone);
+            // the line inputs stored by the set-up (possibly in this same
+            // round's previous pass, for a line that stopped at once) are
+            // complete before they are read back
+            if (D) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
             while (D) {
                 int o[4];
                 take4(D, o);
@@ -348,9 +344,7 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))
                     bool first;
                     line_geom(o_slot, o_line, base, stride, lch, le, soft, first);
                     const int p = base + i * stride;
-                    const double e = first ? 0.0 : le[p];
-                    const double tt = job.alpha * e;
-                    const double xin = lch[p] + tt;
+                    const double xin = le[kCube + 16 * o_line + i]; // the set-up's x
                     double la, lenew;
                     if (ol.nl == 0) { // empty list: L_APP = llr, L_E = 0 (:537-539)
                         la = xin;
@@ -413,6 +407,7 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))
                     const double e = first ? 0.0 : le[p];
                     const double tt = job.alpha * e; // L_A = alpha L_E, then + L_channel (:147-155)
                     const double x = lch[p] + tt;
+                    le[kCube + 16 * o_line + i] = x; // read back by the soft output
                     fin = __builtin_isfinite(x);
                     neg = !(x > 0.0); // HardDec (:758)
                     ai = fabs(x);

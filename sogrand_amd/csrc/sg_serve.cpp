@@ -311,9 +311,10 @@ int sogrand_server_create(sogrand_server **out, sogrand_ctx *ctx, const sogrand_
         s->p = ctx->p;
     }
     s->cfg = c;
-    // per-codeword bytes of a slot: LLRs, L_E scratch, optional soft output,
-    // bits, statistics, status
-    const size_t per_cw = 2 * sg::kCube * sizeof(double) +
+    // per-codeword bytes of a slot: LLRs, L_E scratch (up to two cubes per
+    // codeword for the persistent kernel), optional soft output, bits,
+    // statistics, status
+    const size_t per_cw = (1 + sg::kPersistentSlotScratch) * sg::kCube * sizeof(double) +
                           (c.with_soft ? sg::kCube * sizeof(double) : 0) + sg::kInfoBytes +
                           sizeof(sogrand_cw_stats) + sizeof(int32_t);
     size_t b = c.hbm_budget / c.max_inflight / per_cw;
