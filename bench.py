@@ -390,6 +390,51 @@ def workload_mixed(args) -> dict:
                        "n_cw": n}, "rows": rows}
 
 
+def workload_snr_sweep(args) -> dict:
+    """Both kernels at each Eb/N0 of the plain configuration's sweep, 100,000
+    codewords per point by default: cw/s, error rates and work per codeword."""
+    import sogrand_amd as sg
+    from sogrand_amd import hip
+
+    n = args.n_cw
+    dec = sg.Decoder(0, backend=sg.BACKEND_PERSISTENT)
+    rows = []
+    for k, snr in enumerate((1.0, 1.5, 2.0, 2.5, 3.0)):
+        seed = args.seed + 100 + k
+        d_llr = gen_device(dec, seed, n, sg.sigma_for_ebn0(snr))
+        d_bits = hip.DeviceArray((n, 64), np.uint8)
+        d_st = hip.DeviceArray((n,), sg.CW_STATS_DTYPE)
+        info = sg.gen_info_bits(seed, n)
+        row = {"ebn0_db": snr}
+        ref_bits = None
+        for backend, name in ((sg.BACKEND_PERSISTENT, "persistent"), (sg.BACKEND_PLAIN, "plain")):
+            dec.set_params(backend=backend)
+            wall, kms = timed_decode(dec, d_llr, d_bits, d_st)
+            bits = d_bits.to_host()
+            row[f"{name}_cw_s"] = round(n / wall, 1)
+            if ref_bits is None:
+                ref_bits = bits
+                st = d_st.to_host()
+                be, bl = ber_bler(bits, info)
+                row.update({"ber": be / (512.0 * n), "ber_ci95": wilson(be, 512 * n),
+                            "bler": bl / n, "bler_ci95": wilson(bl, n),
+                            "mean_half_iters": float(st["half_iters"].mean()),
+                            "converged": float(st["converged"].mean()),
+                            "mean_NG": float(st["NG"].astype(np.float64).mean())})
+            else:
+                row["identical_bits"] = bool((bits == ref_bits).all())
+            log(f"{snr} dB {name}: {n / wall:.0f} cw/s")
+        row["batched_over_plain"] = round(row["persistent_cw_s"] / row["plain_cw_s"], 3)
+        rows.append(row)
+        for x in (d_llr, d_bits, d_st):
+            x.free()
+    return {"metric": "codewords/s per Eb/N0, batched and plain",
+            "unit": "codewords/s", "n_gpus": 1, "dtype": "f64",
+            "data": "synthetic (seeded info bits, (16,8)^3 encoding, BPSK, AWGN)",
+            "config": {"workload": f"{n} codewords per Eb/N0 in {{1.0, 1.5, 2.0, 2.5, 3.0}} dB, "
+                                   f"L=3, Imax=30, Tmax unlimited", "n_cw": n}, "rows": rows}
+
+
 def workload_variants(args) -> dict:
     """The reference's three SOGRAND_C decoders on the same kernels: throughput,
     error rates and work of each, 100,000 codewords per Eb/N0 by default."""
@@ -576,8 +621,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="headline",
-                    choices=["headline", "uneven", "mixed-precision", "serving", "variants",
-                             "copy"])
+                    choices=["headline", "snr-sweep", "uneven", "mixed-precision", "serving",
+                             "variants", "copy"])
     ap.add_argument("--n-cw", type=int, default=None)
     ap.add_argument("--ebn0", type=float, default=2.0)
     ap.add_argument("--seed", type=int, default=20260)
@@ -601,7 +646,8 @@ def main() -> int:
         args.n_cw = 100_000
     fn = {"headline": workload_headline, "uneven": workload_uneven,
           "mixed-precision": workload_mixed, "serving": workload_serving,
-          "variants": workload_variants, "copy": workload_copy}[args.workload]
+          "variants": workload_variants, "snr-sweep": workload_snr_sweep,
+          "copy": workload_copy}[args.workload]
     print(json.dumps(fn(args)), flush=True)
     return 0
 
