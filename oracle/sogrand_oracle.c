@@ -23,6 +23,12 @@
  *
  * Transcendentals are glibc's exp/log, as in the reference. Compiled with
  * -ffp-contract=off: no multiply-add is fused.
+ *
+ * Two process-wide switches restate the reference's other decoders:
+ * og_set_epsilon(DBL_EPSILON) gives SOGRAND_C/cubic_decoder_updated.c, and
+ * og_set_variant(1) the earlier SOGRAND_C/cubic_decoder.c (JacLog cut at +-30,
+ * IC guard, P_notGuess never lowered, P_neg of the last guess under a budget,
+ * clamps 1e-9 and 1e-30; see og_siso).
  */
 #define _GNU_SOURCE
 #include "sogrand_oracle.h"
