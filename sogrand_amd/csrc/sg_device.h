@@ -23,14 +23,17 @@
 
 namespace sg {
 
-// One entry of the LDS exp table: 2^(j/32) = hi + lo and the Taylor
-// coefficients 1/6! .. 1/3! of exp(r) - 1. The coefficients are the same in
-// every entry; reading them with the entry keeps them out of loop-invariant
-// registers, which the compiler otherwise pays for with a copy per step.
+// One 16-byte entry of the LDS exp table. Entries 0..31 hold 2^(j/32) = a + b
+// (hi, lo); entries 32 and 33 hold the Taylor coefficients (1/6!, 1/5!) and
+// (1/4!, 1/3!) of exp(r) - 1, read at one address by every lane (a broadcast).
+// Reading the coefficients from LDS keeps them out of loop-invariant
+// registers, which the compiler otherwise pays for with a copy per step; as a
+// separate entry rather than a copy in every entry, the per-lane read at a
+// data-dependent index stays 16 bytes wide.
 struct ExpEntry {
-    double hi, lo;
-    double c6, c5, c4, c3;
+    double a, b;
 };
+constexpr int kExpTableEntries = 34;
 
 struct DecodeCfg {
     uint64_t tmax;          // 0 already mapped to 0x7fffffff by the host
@@ -43,42 +46,41 @@ struct DecodeCfg {
                             // cubic_decoder.c:533)
 };
 
-// Fill the LDS exp table (first 32 threads) - the caller synchronises.
+// Fill the LDS exp table (first 34 threads) - the caller synchronises.
 __device__ __forceinline__ void load_exp_table(ExpEntry *lds, int tid)
 {
-    if (tid < 32)
-        lds[tid] = ExpEntry{kExp2Table[tid].hi, kExp2Table[tid].lo, 1.0 / 720.0, 1.0 / 120.0,
-                            1.0 / 24.0, 1.0 / 6.0};
+    if (tid < 32) lds[tid] = ExpEntry{kExp2Table[tid].hi, kExp2Table[tid].lo};
+    if (tid == 32) lds[32] = ExpEntry{1.0 / 720.0, 1.0 / 120.0};
+    if (tid == 33) lds[33] = ExpEntry{1.0 / 24.0, 1.0 / 6.0};
 }
 
 // exp(-pm) for pm >= 0, the probability weight of one guess (:626, :672, :712).
 // Tang-style: -pm = (32k + j) ln2/32 + r with |r| <= ln2/64, exp(r) - 1 = q by
 // its Taylor polynomial to r^6, and 2^(j/32) = hi + lo from the LDS table; the
 // result hi + (hi q + lo (1 + q)) scaled by 2^k. Within ~0.5 ulp (it differs
-// from the correctly rounded value on about 1 input in 200). Underflows to 0
-// beyond pm = 745.5, as exp does.
+// from the correctly rounded value on about 1 input in 200). Results below
+// 2^-1075 (pm beyond about 745.13) round to 0 in the final ldexp, as exp's do.
+// pm is capped at 746 (which also maps +inf there) to keep k in int range: at
+// the cap k >> 5 = -1077 and m < 2, so the result is 0 as well.
 __device__ __forceinline__ double exp_neg(double pm, const ExpEntry *tab)
 {
-    const bool under = !(pm < 745.5);
-    const double x = -pm;
-    // the bound only keeps k in int range for pm >= 745.5 (and NaN), whose
-    // result is replaced by 0 below; below 745.5 it is never active
-    const double kd = __builtin_fmax(__builtin_rint(x * kExpInvLn2N), -34432.0);
+    const double x = -__builtin_fmin(pm, 746.0);
+    const double kd = __builtin_rint(x * kExpInvLn2N);
     const int k = (int)kd;
-    const ExpEntry &t = tab[k & 31];
+    const ExpEntry t = tab[k & 31];
+    const ExpEntry c65 = tab[32], c43 = tab[33];
     double r = __builtin_fma(-kd, kExpLn2NHi, x);
     r = __builtin_fma(-kd, kExpLn2NLo, r);
-    double q = t.c6;
-    q = __builtin_fma(q, r, t.c5);
-    q = __builtin_fma(q, r, t.c4);
-    q = __builtin_fma(q, r, t.c3);
+    double q = c65.a;
+    q = __builtin_fma(q, r, c65.b);
+    q = __builtin_fma(q, r, c43.a);
+    q = __builtin_fma(q, r, c43.b);
     q = __builtin_fma(q, r, 0.5);
     q = __builtin_fma(q, r, 1.0);
     q = q * r;
-    const double s = __builtin_fma(t.lo, q, t.lo);
-    const double m = __builtin_fma(t.hi, q, s) + t.hi;
-    const double v = __builtin_ldexp(m, k >> 5);
-    return under ? 0.0 : v;
+    const double s = __builtin_fma(t.b, q, t.b);
+    const double m = __builtin_fma(t.a, q, s) + t.a;
+    return __builtin_ldexp(m, k >> 5);
 }
 
 // Arithmetic of the per-guess path metrics.

@@ -85,7 +85,7 @@ __global__ __launch_bounds__(kPlainThreads) void plain_kernel(CubeJob job)
     __shared__ uint16_t s_hp[17 * kPlainThreads];
     __shared__ uint8_t s_hd[kCube];
     __shared__ uint32_t s_sum[kPlainThreads / 64], s_max[kPlainThreads / 64];
-    __shared__ ExpEntry s_exp[32];
+    __shared__ ExpEntry s_exp[kExpTableEntries];
 
     const int t = threadIdx.x;
     load_exp_table(s_exp, t);
@@ -200,7 +200,7 @@ __global__ __launch_bounds__(256) void lines_kernel(const double *llr, size_t n,
     using Rec = typename P::Rec;
     __shared__ Rec s_delta[17 * 256];
     __shared__ uint16_t s_hp[17 * 256];
-    __shared__ ExpEntry s_exp[32];
+    __shared__ ExpEntry s_exp[kExpTableEntries];
     load_exp_table(s_exp, threadIdx.x);
     __syncthreads();
     cfg.exptab = s_exp;

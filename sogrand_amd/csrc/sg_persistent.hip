@@ -146,7 +146,7 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))
     __shared__ double s_stage_jn[kWaves][64];
     __shared__ double s_stage_a[kWaves][4][2]; // per sub-round line: |x| of ranks 0 and 7
     __shared__ uint32_t s_geom[kWaves][S][2];  // per slot: cube, axis | iteration << 8
-    __shared__ ExpEntry s_exp[32];
+    __shared__ ExpEntry s_exp[kExpTableEntries];
 
     load_exp_table(s_exp, threadIdx.x);
     __syncthreads(); // the only workgroup barrier: waves run independently from here
