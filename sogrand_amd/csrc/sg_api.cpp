@@ -213,16 +213,18 @@ int sogrand_ctx_create(sogrand_ctx **out, int device, const sogrand_params *para
         sogrand_ctx_destroy(c);
         return SOGRAND_EHIP;
     }
-    // one zero chunk of padding after the last table: the search loop
-    // prefetches one 16-byte chunk past the end marker
-    std::vector<uint16_t> tables((size_t)sg::kSchedTables * sg::kSchedStride + 8, 0);
-    sg::build_all_schedule_tables(tables.data());
-    if (hipMalloc(&c->d_sched, tables.size() * sizeof(uint16_t)) != hipSuccess) {
+    // mask tables and position-word tables in one buffer (sg_common.h)
+    std::vector<uint8_t> buf(sg::kSchedBufferBytes, 0);
+    uint16_t *masks = reinterpret_cast<uint16_t *>(buf.data());
+    uint32_t *words = reinterpret_cast<uint32_t *>(buf.data() + sg::kSchedPosOffsetBytes);
+    sg::build_all_schedule_tables(masks);
+    for (size_t i = 0; i < (size_t)sg::kSchedTables * sg::kSchedStride; ++i)
+        words[i] = sg::pos_word(masks[i]);
+    if (hipMalloc(&c->d_sched, buf.size()) != hipSuccess) {
         sogrand_ctx_destroy(c);
         return SOGRAND_ENOMEM;
     }
-    if (hipMemcpy(c->d_sched, tables.data(), tables.size() * sizeof(uint16_t),
-                  hipMemcpyHostToDevice) != hipSuccess) {
+    if (hipMemcpy(c->d_sched, buf.data(), buf.size(), hipMemcpyHostToDevice) != hipSuccess) {
         sogrand_ctx_destroy(c);
         return SOGRAND_EHIP;
     }

@@ -123,6 +123,36 @@ constexpr int kSchedStride = 65536; // table stride (entry 65535 = 0, end marker
 constexpr int kSchedTables = 56;    // IC = 0..54 and one generic table for IC >= 55
 constexpr int kSchedGenericIc = 55;
 
+// The same schedules as position words, one u32 per entry: bits 5i .. 5i + 4
+// hold b + 1 for the i-th lowest set bit b of the mask (i = 0..5), 0 where the
+// mask has fewer bits, and bit 30 is set when it has more than six (the rest
+// are then read from the mask). The end marker (mask 0) is the word 0. A field
+// is a record row of the line's LDS records (row 0 the null record), so a
+// pattern's positions come out with one bit-field extract each.
+constexpr uint32_t kPosFields = 6;
+constexpr uint32_t kPosMore = 1u << 30;
+inline constexpr uint32_t pos_word(uint32_t m)
+{
+    uint32_t w = 0;
+    for (uint32_t i = 0; i < kPosFields && m; ++i) {
+        uint32_t b = 0;
+        while (!((m >> b) & 1u)) ++b;
+        w |= (b + 1u) << (5u * i);
+        m &= m - 1u;
+    }
+    return m ? (w | kPosMore) : w;
+}
+static_assert(pos_word(0x1u) == 1u && pos_word(0x8001u) == (1u | (16u << 5)), "position word");
+static_assert(pos_word(0x7fu) == (kPosMore | 1u | 2u << 5 | 3u << 10 | 4u << 15 | 5u << 20 | 6u << 25),
+              "position word overflow");
+// Device layout of the schedule buffer: the mask tables, 8 zero masks of
+// padding, then the position-word tables and 8 zero words of padding (the
+// search loops prefetch one chunk past an end marker).
+constexpr size_t kSchedMaskEntries = (size_t)kSchedTables * kSchedStride + 8;
+constexpr size_t kSchedPosOffsetBytes = kSchedMaskEntries * 2;
+constexpr size_t kSchedBufferBytes = kSchedPosOffsetBytes + ((size_t)kSchedTables * kSchedStride + 8) * 4;
+static_assert(kSchedPosOffsetBytes % 16 == 0, "position tables are 16-byte aligned");
+
 // ---------------------------------------------------------------------------
 // Seeded synthetic data. Counter-based hashing (splitmix64 finaliser) so that
 // codeword c depends only on (seed, c); a portable log and sin/cos built from

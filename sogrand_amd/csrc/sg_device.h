@@ -123,9 +123,10 @@ __device__ __forceinline__ double exp_neg_f32(float pm)
 
 // Per-lane LDS record columns of an NS-slot array: delta (Rec) and (H column |
 // natural position << 8) of sorted positions b = 0..15 at rows 1..16, and at
-// row 0 a null record (delta +0.0, H column 0) that the pattern loop reads as
-// position -1, the index v_ffbl gives an exhausted mask. The column pointers
-// fold the slot in, so a record address is one shift-add of b.
+// row 0 a null record (delta +0.0, H column 0) that the pattern loops read as
+// position -1, the index v_ffbl gives an exhausted mask, or as the empty field
+// (row 0) of a position word. The column pointers fold the slot in, so a
+// record address is one shift-add of b.
 template <class RecT, int NS> struct LineLdsT {
     using Rec = RecT;
     Rec *dcol;       // this slot's record of sorted position 0
@@ -136,6 +137,10 @@ template <class RecT, int NS> struct LineLdsT {
     }
     __device__ Rec &d(int b) const { return dcol[b * NS]; }
     __device__ uint16_t &h(int b) const { return hcol[b * NS]; }
+    // the same records by row (row = b + 1, row 0 the null record), as the
+    // fields of a position word give them
+    __device__ Rec dr(uint32_t row) const { return dcol[((int)row - 1) * NS]; }
+    __device__ uint16_t hr(uint32_t row) const { return hcol[((int)row - 1) * NS]; }
 };
 
 // Index of the lowest set bit, -1 for 0 (v_ffbl_b32). Written as the
@@ -509,13 +514,24 @@ struct SearchState {
     uint32_t t;       // next schedule index
     double pnot;      // P_notGuess (:614, :626, :672, :712)
     bool done;
-    const uint4 *tp;  // schedule table of the line's intercept
-    uint4 nxt;        // 16-byte chunk of the next 8 masks (prefetched)
+    const uint4 *tp;  // schedule table of the line's intercept: masks, or
+                      // position words in the unbudgeted search
+    uint4 nxt;        // the next chunk of 8 masks (prefetched), or the first
+    uint4 nxt2;       // and second halves of the next 8 position words
     int64_t cur_wt;   // weight level of the last pattern (finite Tmax only)
 };
 
-// Hard-decision guess (:624-641) and schedule selection.
-template <class P>
+// The position-word tables, at a fixed offset behind the mask tables in the
+// one schedule buffer.
+__device__ __forceinline__ const uint32_t *pos_tables(const uint16_t *sched)
+{
+    return reinterpret_cast<const uint32_t *>(reinterpret_cast<const uint8_t *>(sched) +
+                                              kSchedPosOffsetBytes);
+}
+
+// Hard-decision guess (:624-641) and schedule selection (position words for
+// the unbudgeted search, masks under a guess budget).
+template <class P, bool FINITE_T>
 __device__ __forceinline__ void search_init(SearchState &ss, LineList &ls, uint32_t hd, uint32_t s0,
                                             double pm_hd, int32_t ic, const DecodeCfg &cfg)
 {
@@ -536,10 +552,66 @@ __device__ __forceinline__ void search_init(SearchState &ss, LineList &ls, uint3
     ss.done = false;
     if (s0 == 0) ss.done = list_hit(ls, e_hd, 0u, ss.pnot, cfg);
     const int tbl = ic < kSchedGenericIc ? ic : kSchedGenericIc;
-    ss.tp = reinterpret_cast<const uint4 *>(cfg.sched + (size_t)tbl * kSchedStride);
-    ss.nxt = ss.tp[0];
+    if constexpr (FINITE_T) {
+        ss.tp = reinterpret_cast<const uint4 *>(cfg.sched + (size_t)tbl * kSchedStride);
+        ss.nxt = ss.tp[0];
+    } else {
+        ss.tp = reinterpret_cast<const uint4 *>(pos_tables(cfg.sched) + (size_t)tbl * kSchedStride);
+        ss.nxt = ss.tp[0];
+        ss.nxt2 = ss.tp[1];
+    }
     ss.t = 0;
     ss.cur_wt = -1;
+}
+
+// Accounting of a pair of guesses at schedule positions t and t + 1 from their
+// path metrics and syndromes, strictly in schedule order (:652-744): T,
+// P_notGuess, list insertion and the stopping rule; the second guess is
+// dropped if the first ends the search or take2 is false. mask_of(j) gives
+// the mask of guess j where a hit's metric is summed afterwards (the earlier
+// decoder without a budget).
+template <class P, bool kPmLoop, class LDS, class MaskOf>
+__device__ __forceinline__ void pair_account(SearchState &ss, LineList &ls, const LDS &S,
+                                             double pm_hd, const DecodeCfg &cfg,
+                                             typename P::Rec pm1, typename P::Rec pm2,
+                                             uint32_t syn1, uint32_t syn2, bool take2,
+                                             uint32_t t, MaskOf mask_of)
+{
+    const bool hit1 = (syn1 & 0xffu) == 0, hit2 = (syn2 & 0xffu) == 0;
+    double e1 = 0.0, e2 = 0.0;
+    if constexpr (P::kF32) {
+        // single-precision weights for the found-mass chain; a hit's list
+        // weight is taken in double from its (single-precision) metric, in the
+        // (rare) hit branch below
+        e1 = exp_neg_f32((float)pm1);
+        e2 = exp_neg_f32((float)pm2);
+    } else if constexpr (kPmLoop) {
+        e1 = exp_neg((double)pm1, cfg.exptab);
+        e2 = exp_neg((double)pm2, cfg.exptab);
+    }
+    ss.T += 1;
+    ss.t = t + 1;
+    if (P::kF32 && hit1) e1 = exp_neg((double)pm1, cfg.exptab);
+    if (!kPmLoop && hit1) e1 = exp_neg(pattern_pm<P>(S, mask_of(0), pm_hd), cfg.exptab);
+    if constexpr (!P::kLegacy)
+        ss.pnot -= e1;
+    else if (kPmLoop && !hit1)
+        ls.pnl = e1; // cubic_decoder.c:638
+    bool stop = hit1 && list_hit(ls, e1, t + 1u, ss.pnot, cfg);
+    // end of the schedule, or the next weight level is over budget
+    if (!stop && !take2) stop = true;
+    if (!stop) {
+        ss.T += 1;
+        ss.t = t + 2;
+        if (P::kF32 && hit2) e2 = exp_neg((double)pm2, cfg.exptab);
+        if (!kPmLoop && hit2) e2 = exp_neg(pattern_pm<P>(S, mask_of(1), pm_hd), cfg.exptab);
+        if constexpr (!P::kLegacy)
+            ss.pnot -= e2;
+        else if (kPmLoop && !hit2)
+            ls.pnl = e2;
+        stop = hit2 && list_hit(ls, e2, t + 2u, ss.pnot, cfg);
+    }
+    ss.done = stop;
 }
 
 // Two consecutive guesses of the schedule (:652-744): the patterns of `pair`
@@ -585,7 +657,7 @@ __device__ __forceinline__ void search_pair(SearchState &ss, LineList &ls, const
     // both sums unchanged, so no lane needs a select or a branch
     // (the first round reads the hard decision's metric and syndrome directly)
     const Rec pm_base = (Rec)pm_hd;
-    Rec pm1, pm2;
+    Rec pm1 = 0, pm2 = 0;
     uint32_t syn1, syn2;
     uint32_t r1 = m1, r2 = take2 ? m2 : 0u;
     {
@@ -616,45 +688,91 @@ __device__ __forceinline__ void search_pair(SearchState &ss, LineList &ls, const
         r1 = q1 & (q1 - 1u);
         r2 = q2 & (q2 - 1u);
     }
-    const bool hit1 = (syn1 & 0xffu) == 0, hit2 = (syn2 & 0xffu) == 0;
-    double e1 = 0.0, e2 = 0.0;
-    if constexpr (P::kF32) {
-        // single-precision weights for the found-mass chain; a hit's list
-        // weight is taken in double from its (single-precision) metric, in the
-        // (rare) hit branch below
-        e1 = exp_neg_f32((float)pm1);
-        e2 = exp_neg_f32((float)pm2);
-    } else if constexpr (kPmLoop) {
-        e1 = exp_neg((double)pm1, cfg.exptab);
-        e2 = exp_neg((double)pm2, cfg.exptab);
-    }
-    ss.T += 1;
-    ss.t = t + 1;
-    if (P::kF32 && hit1) e1 = exp_neg((double)pm1, cfg.exptab);
-    if (!kPmLoop && hit1) e1 = exp_neg(pattern_pm<P>(S, m1, pm_hd), cfg.exptab);
-    if constexpr (!P::kLegacy)
-        ss.pnot -= e1;
-    else if (kPmLoop && !hit1)
-        ls.pnl = e1; // cubic_decoder.c:638
-    bool stop = hit1 && list_hit(ls, e1, t + 1u, ss.pnot, cfg);
-    // end of the schedule, or the next weight level is over budget
-    if (!stop && !take2) stop = true;
-    if (!stop) {
-        ss.T += 1;
-        ss.t = t + 2;
-        if (P::kF32 && hit2) e2 = exp_neg((double)pm2, cfg.exptab);
-        if (!kPmLoop && hit2) e2 = exp_neg(pattern_pm<P>(S, m2, pm_hd), cfg.exptab);
-        if constexpr (!P::kLegacy)
-            ss.pnot -= e2;
-        else if (kPmLoop && !hit2)
-            ls.pnl = e2;
-        stop = hit2 && list_hit(ls, e2, t + 2u, ss.pnot, cfg);
-    }
-    ss.done = stop;
+    pair_account<P, kPmLoop>(ss, ls, S, pm_hd, cfg, pm1, pm2, syn1, syn2, take2, t,
+                             [&](int j) { return j == 0 ? m1 : m2; });
 }
 
-// Further guesses of the schedule, one 16-byte chunk of 8 masks (four pairs)
-// per iteration, up to about max_steps guesses. A search starts at a chunk
+template <int I> struct IntC {
+    static constexpr int value = I;
+};
+
+// Field I of a position word (v_bfe_u32, written as the instruction: as a C
+// expression the compiler folds the extract into the address scaling, one
+// instruction more per field).
+template <int I> __device__ __forceinline__ uint32_t pos_field(uint32_t w)
+{
+    uint32_t f;
+    asm("v_bfe_u32 %0, %1, %2, 5" : "=v"(f) : "v"(w), "i"(5 * I));
+    return f;
+}
+
+// The same pair of guesses from position words (the unbudgeted search): the
+// fields of both words are read two at a time; the rounds past the first take
+// a wave-uniform branch, so a round runs only if some lane's pattern has that
+// many set bits, and a lane whose pattern has fewer reads the null record
+// (row 0), which leaves both sums unchanged. Bits past the sixth (rare) come
+// from the mask. The sums run in ascending sorted position, as in search_pair.
+template <class P, class LDS>
+__device__ __forceinline__ void search_pair_pos(SearchState &ss, LineList &ls, const LDS &S,
+                                                uint32_t s0, double pm_hd, int32_t ic,
+                                                const DecodeCfg &cfg, uint32_t w1, uint32_t w2,
+                                                uint32_t t)
+{
+    using Rec = typename P::Rec;
+    constexpr bool kPmLoop = !P::kLegacy;
+    // the end marker is the word 0; every other word has a nonzero first field
+    const bool take2 = w2 != 0;
+    const Rec pm_base = (Rec)pm_hd;
+    Rec pm1 = 0, pm2 = 0;
+    uint32_t syn1, syn2;
+    {
+        const uint32_t a1 = pos_field<0>(w1), b1 = pos_field<1>(w1);
+        const uint32_t a2 = pos_field<0>(w2), b2 = pos_field<1>(w2);
+        syn1 = s0 ^ (uint32_t)S.hr(a1) ^ (uint32_t)S.hr(b1);
+        syn2 = s0 ^ (uint32_t)S.hr(a2) ^ (uint32_t)S.hr(b2);
+        if constexpr (kPmLoop) {
+            pm1 = (pm_base + S.dr(a1)) + S.dr(b1);
+            pm2 = (pm_base + S.dr(a2)) + S.dr(b2);
+        }
+    }
+    auto round = [&](auto i0) {
+        constexpr int I = decltype(i0)::value;
+        const uint32_t a1 = pos_field<I>(w1), b1 = pos_field<I + 1>(w1);
+        const uint32_t a2 = pos_field<I>(w2), b2 = pos_field<I + 1>(w2);
+        syn1 ^= (uint32_t)S.hr(a1) ^ (uint32_t)S.hr(b1);
+        syn2 ^= (uint32_t)S.hr(a2) ^ (uint32_t)S.hr(b2);
+        if constexpr (kPmLoop) {
+            pm1 = (pm1 + S.dr(a1)) + S.dr(b1);
+            pm2 = (pm2 + S.dr(a2)) + S.dr(b2);
+        }
+    };
+    if (__ballot(((w1 | w2) >> 10) != 0u) != 0ull) {
+        round(IntC<2>{});
+        if (__ballot(((w1 | w2) >> 20) != 0u) != 0ull) round(IntC<4>{});
+    }
+    const int tbl = ic < kSchedGenericIc ? ic : kSchedGenericIc;
+    const uint16_t *masks = cfg.sched + (size_t)tbl * kSchedStride;
+    if (__ballot(((w1 | w2) & kPosMore) != 0u) != 0ull) {
+        auto more = [&](uint32_t m, Rec &pm, uint32_t &syn) {
+#pragma unroll 1
+            for (uint32_t i = 0; i < kPosFields; ++i) m &= m - 1u;
+#pragma unroll 1
+            for (; m; m &= m - 1u) {
+                const int b = __builtin_ctz(m);
+                syn ^= (uint32_t)S.h(b);
+                if constexpr (kPmLoop) pm = pm + S.d(b);
+            }
+        };
+        if (w1 & kPosMore) more(masks[t], pm1, syn1);
+        if (w2 & kPosMore) more(masks[t + 1u], pm2, syn2);
+    }
+    pair_account<P, kPmLoop>(ss, ls, S, pm_hd, cfg, pm1, pm2, syn1, syn2, take2, t,
+                             [&](int j) { return (uint32_t)masks[t + (uint32_t)j]; });
+}
+
+// Further guesses of the schedule, one chunk of 8 entries (four pairs: 16
+// bytes of masks, or 32 bytes of position words without a guess budget) per
+// iteration, up to about max_steps guesses. A search starts at a chunk
 // boundary and a lane that stops mid-chunk is done, so every searching lane
 // enters an iteration at a chunk boundary; the chunk was loaded one iteration
 // ahead. The loop runs an iteration count common to the wave and leaves when
@@ -665,6 +783,32 @@ __device__ __forceinline__ void search_run(SearchState &ss, LineList &ls, const 
                                            uint32_t s0, double pm_hd, int32_t ic,
                                            const DecodeCfg &cfg, uint32_t max_steps)
 {
+    if constexpr (!FINITE_T) {
+        // position words: a chunk of 8 is two 16-byte halves
+#pragma unroll 1
+        for (uint32_t step = 0; step < max_steps; step += 8) {
+            if (__ballot(!ss.done) == 0ull) break;
+            if (ss.done) continue;
+            const uint32_t t = ss.t; // a multiple of 8 here
+            uint4 c, d;
+            asm volatile("v_mov_b32 %0, %1" : "=v"(c.x) : "v"(ss.nxt.x) : "memory");
+            asm volatile("v_mov_b32 %0, %1" : "=v"(c.y) : "v"(ss.nxt.y) : "memory");
+            asm volatile("v_mov_b32 %0, %1" : "=v"(c.z) : "v"(ss.nxt.z) : "memory");
+            asm volatile("v_mov_b32 %0, %1" : "=v"(c.w) : "v"(ss.nxt.w) : "memory");
+            asm volatile("v_mov_b32 %0, %1" : "=v"(d.x) : "v"(ss.nxt2.x) : "memory");
+            asm volatile("v_mov_b32 %0, %1" : "=v"(d.y) : "v"(ss.nxt2.y) : "memory");
+            asm volatile("v_mov_b32 %0, %1" : "=v"(d.z) : "v"(ss.nxt2.z) : "memory");
+            asm volatile("v_mov_b32 %0, %1" : "=v"(d.w) : "v"(ss.nxt2.w) : "memory");
+            const uint4 *np = ss.tp + (t >> 2) + 2; // tables carry one chunk of padding
+            ss.nxt = np[0];
+            ss.nxt2 = np[1];
+            search_pair_pos<P>(ss, ls, S, s0, pm_hd, ic, cfg, c.x, c.y, t);
+            if (!ss.done) search_pair_pos<P>(ss, ls, S, s0, pm_hd, ic, cfg, c.z, c.w, t + 2u);
+            if (!ss.done) search_pair_pos<P>(ss, ls, S, s0, pm_hd, ic, cfg, d.x, d.y, t + 4u);
+            if (!ss.done) search_pair_pos<P>(ss, ls, S, s0, pm_hd, ic, cfg, d.z, d.w, t + 6u);
+        }
+        return;
+    }
 #pragma unroll 1
     for (uint32_t step = 0; step < max_steps; step += 8) {
         if (__ballot(!ss.done) == 0ull) break;
@@ -694,7 +838,7 @@ __device__ __forceinline__ void line_search(const LDS &S, uint32_t hd,
                                             const DecodeCfg &cfg, LineList &ls)
 {
     SearchState ss;
-    search_init<P>(ss, ls, hd, s0, pm_hd, ic, cfg);
+    search_init<P, FINITE_T>(ss, ls, hd, s0, pm_hd, ic, cfg);
     search_run<FINITE_T, P>(ss, ls, S, hd, s0, pm_hd, ic, cfg, 0xffffffffu);
     ls.T = ss.T;
 }
