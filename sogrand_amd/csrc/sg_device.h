@@ -64,7 +64,12 @@ __device__ __forceinline__ void load_exp_table(ExpEntry *lds, int tid)
 // the cap k >> 5 = -1077 and m < 2, so the result is 0 as well.
 __device__ __forceinline__ double exp_neg(double pm, const ExpEntry *tab)
 {
-    const double x = -__builtin_fmin(pm, 746.0);
+    // the cap as the instruction itself: as fmin the compiler first quiets a
+    // possible signalling NaN (one more instruction); pm is a sum of finite or
+    // infinite non-negative terms, never NaN
+    double pc;
+    asm("v_min_f64 %0, %1, %2" : "=v"(pc) : "v"(pm), "v"(746.0));
+    const double x = -pc;
     const double kd = __builtin_rint(x * kExpInvLn2N);
     const int k = (int)kd;
     const ExpEntry t = tab[k & 31];
@@ -510,8 +515,8 @@ __device__ __forceinline__ int position_sum(uint32_t m)
 // State of one line's guessing loop (sogrand_main_logic :624-744), so that
 // the loop can be run in bounded steps.
 struct SearchState {
-    uint32_t T;       // guesses so far
-    uint32_t t;       // next schedule index
+    uint32_t t;       // next schedule index; the guesses so far are t + 1 (the
+                      // hard decision and t patterns)
     double pnot;      // P_notGuess (:614, :626, :672, :712)
     bool done;
     const uint4 *tp;  // schedule table of the line's intercept: masks, or
@@ -539,7 +544,6 @@ __device__ __forceinline__ void search_init(SearchState &ss, LineList &ls, uint3
     ls.pnl = 0.0;
     ls.ppos = 0.0;
     ls.code = 0;
-    ss.T = 1;
     const double e_hd = exp_neg(pm_hd, cfg.exptab);
     if (P::kLegacy) {
         // cubic_decoder.c:590: P_notGuess stays 1; P_neg holds the last
@@ -589,7 +593,6 @@ __device__ __forceinline__ void pair_account(SearchState &ss, LineList &ls, cons
         e1 = exp_neg((double)pm1, cfg.exptab);
         e2 = exp_neg((double)pm2, cfg.exptab);
     }
-    ss.T += 1;
     ss.t = t + 1;
     if (P::kF32 && hit1) e1 = exp_neg((double)pm1, cfg.exptab);
     if (!kPmLoop && hit1) e1 = exp_neg(pattern_pm<P>(S, mask_of(0), pm_hd), cfg.exptab);
@@ -601,7 +604,6 @@ __device__ __forceinline__ void pair_account(SearchState &ss, LineList &ls, cons
     // end of the schedule, or the next weight level is over budget
     if (!stop && !take2) stop = true;
     if (!stop) {
-        ss.T += 1;
         ss.t = t + 2;
         if (P::kF32 && hit2) e2 = exp_neg((double)pm2, cfg.exptab);
         if (!kPmLoop && hit2) e2 = exp_neg(pattern_pm<P>(S, mask_of(1), pm_hd), cfg.exptab);
@@ -642,13 +644,13 @@ __device__ __forceinline__ void search_pair(SearchState &ss, LineList &ls, const
         const int64_t wt1 = (int64_t)ic * __builtin_popcount(m1) + position_sum(m1);
         const int64_t wt2 = (int64_t)ic * __builtin_popcount(m2) + position_sum(m2);
         if (wt1 != ss.cur_wt) {
-            if (!((uint64_t)ss.T < cfg.tmax)) {
+            if (!((uint64_t)ss.t + 1u < cfg.tmax)) { // T < Tmax, T = t + 1
                 ss.done = true;
                 return;
             }
             ss.cur_wt = wt1;
         }
-        if (take2 && wt2 != wt1 && !((uint64_t)ss.T + 1u < cfg.tmax)) take2 = false;
+        if (take2 && wt2 != wt1 && !((uint64_t)ss.t + 2u < cfg.tmax)) take2 = false;
         else if (take2) ss.cur_wt = wt2;
     }
     // path metrics (ascending sorted position, as getPM adds them) and
@@ -840,7 +842,7 @@ __device__ __forceinline__ void line_search(const LDS &S, uint32_t hd,
     SearchState ss;
     search_init<P, FINITE_T>(ss, ls, hd, s0, pm_hd, ic, cfg);
     search_run<FINITE_T, P>(ss, ls, S, hd, s0, pm_hd, ic, cfg, 0xffffffffu);
-    ls.T = ss.T;
+    ls.T = ss.t + 1u;
 }
 
 // Soft output of position i (SOGRAND_bitSO :536-575) given the line input x.

@@ -181,7 +181,7 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))
     SearchState ss;
     ls.nl = 0;
     ls.pnl = 0.0;
-    ss.T = 0;
+    ss.t = ~0u; // no line yet: t + 1 = 0 guesses
     ss.done = true;
 
     // geometry of (slot, line) as seen by any lane
@@ -359,8 +359,8 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))
                     if (!(la > 0.0)) atomicOr(&hd_bits[128 * o_slot + (p >> 5)], 1u << (p & 31));
                 }
                 if (lane == o[0] || lane == o[1] || lane == o[2] || lane == o[3]) {
-                    atomicAdd(&cnt[4 * my_slot + 1], ss.T);
-                    atomicMax(&cnt[4 * my_slot + 2], ss.T);
+                    atomicAdd(&cnt[4 * my_slot + 1], ss.t + 1u);
+                    atomicMax(&cnt[4 * my_slot + 2], ss.t + 1u);
                     atomicAdd(&cnt[4 * my_slot + 0], 1u);
                     state = kIdle;
                 }
@@ -468,7 +468,7 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))
                         ls.pnl = 0.0;
                         ls.ppos = 0.0;
                         ls.code = 0;
-                        ss.T = 0;
+                        ss.t = ~0u; // not decoded: t + 1 = 0 guesses
                         ss.done = true;
                         state = kDone;
                     } else {
