@@ -18,6 +18,7 @@
 
 #include "sg_common.h"
 #include "sg_exp_table.h"
+#include "sg_log.h"
 
 #pragma clang fp contract(off)
 
@@ -41,6 +42,7 @@ struct DecodeCfg {
     const uint16_t *sched;  // kSchedTables x kSchedStride masks
     uint32_t L;             // list size 1..kMaxL
     const ExpEntry *exptab; // exp table in LDS (load_exp_table)
+    const LogEntry *logtab; // log table in LDS (load_log_table)
     double eps;             // EPSILON of the confidence test, P_notinlist floor and clamps
     double eps_log;         // floor inside the soft output's logs (eps, or 1e-30 for
                             // cubic_decoder.c:533)
@@ -52,6 +54,13 @@ __device__ __forceinline__ void load_exp_table(ExpEntry *lds, int tid)
     if (tid < 32) lds[tid] = ExpEntry{kExp2Table[tid].hi, kExp2Table[tid].lo};
     if (tid == 32) lds[32] = ExpEntry{1.0 / 720.0, 1.0 / 120.0};
     if (tid == 33) lds[33] = ExpEntry{1.0 / 24.0, 1.0 / 6.0};
+}
+
+// Fill the LDS log table (first kLogTableEntries threads) - the caller
+// synchronises.
+__device__ __forceinline__ void load_log_table(LogEntry *lds, int tid)
+{
+    if (tid < kLogTableEntries) lds[tid] = kLogTable[tid];
 }
 
 // exp(-pm) for pm >= 0, the probability weight of one guess (:626, :672, :712).
@@ -159,11 +168,11 @@ __device__ __forceinline__ int lowest_bit(uint32_t x)
 }
 
 // JacLog (:762; cut at lim = 50, or 30 in cubic_decoder.c:699).
-__device__ __forceinline__ double jaclog(double x, double lim)
+__device__ __forceinline__ double jaclog(double x, double lim, const LogEntry *lt)
 {
     if (x > lim) return x;
     if (x < -lim) return 0.0;
-    return log(1.0 + exp(x));
+    return log_pos(1.0 + exp(x), lt);
 }
 
 
@@ -247,7 +256,8 @@ struct ListEntries {
 // order: the reference's quicksort decides the order among equal magnitudes
 // (replayed by quicksort_lds), then PM_HD in natural order and the sorted
 // records. Returns PM_HD.
-template <class LDS> __device__ __forceinline__ double setup_with_ties(const LDS S, double jlim)
+template <class LDS>
+__device__ __forceinline__ double setup_with_ties(const LDS S, double jlim, const LogEntry *lt)
 {
 #pragma unroll 1
     for (int i = 0; i < 16; ++i) S.h(i) = (uint16_t)(hcol_of((unsigned)i) | (i << 8));
@@ -259,11 +269,11 @@ template <class LDS> __device__ __forceinline__ double setup_with_ties(const LDS
     double pm_hd = 0.0;
 #pragma unroll 1
     for (int i = 0; i < 16; ++i)
-        pm_hd += jaclog(-S.d((int)((slot_of >> (4 * i)) & 15u)), jlim);
+        pm_hd += jaclog(-S.d((int)((slot_of >> (4 * i)) & 15u)), jlim, lt);
 #pragma unroll 1
     for (int b = 0; b < 16; ++b) {
         const double ab = S.d(b);
-        S.d(b) = jaclog(ab, jlim) - jaclog(-ab, jlim);
+        S.d(b) = jaclog(ab, jlim, lt) - jaclog(-ab, jlim, lt);
     }
     return pm_hd;
 }
@@ -317,7 +327,7 @@ template <class LDS> __device__ __forceinline__ void place_by_rank(const LDS &S,
 template <class Src, class LDS>
 __device__ __forceinline__ bool line_setup(const Src &src, const LDS &S, uint32_t &hd,
                                            uint32_t &s0, double &pm_hd, int32_t &ic,
-                                           bool legacy)
+                                           bool legacy, const LogEntry *lt)
 {
     const double jlim = legacy ? 30.0 : 50.0;
     bool finite = true;
@@ -361,13 +371,13 @@ __device__ __forceinline__ bool line_setup(const Src &src, const LDS &S, uint32_
 #pragma unroll 1
         for (int i = 0; i < 16; ++i) {
             const double ai = S.d(i);
-            const double jn = jaclog(-ai, jlim);
+            const double jn = jaclog(-ai, jlim, lt);
             pm_hd += jn;
-            S.d(i) = jaclog(ai, jlim) - jn;
+            S.d(i) = jaclog(ai, jlim, lt) - jn;
         }
         place_by_rank(S, rank);
     } else {
-        pm_hd = setup_with_ties(S, jlim);
+        pm_hd = setup_with_ties(S, jlim, lt);
     }
     S.d(-1) = 0.0; // null record read by exhausted masks in search_run
     S.h(-1) = 0;
@@ -381,7 +391,7 @@ __device__ __forceinline__ bool line_setup(const Src &src, const LDS &S, uint32_
 template <class Src, class LDS>
 __device__ __forceinline__ bool line_setup_f32(const Src &src, const LDS &S,
                                                uint32_t &hd, uint32_t &s0, double &pm_hd,
-                                               int32_t &ic)
+                                               int32_t &ic, const LogEntry *lt)
 {
     bool finite = true;
     hd = 0;
@@ -414,9 +424,9 @@ __device__ __forceinline__ bool line_setup_f32(const Src &src, const LDS &S,
 #pragma unroll 1
     for (int i = 0; i < 16; ++i) {
         const double ai = fabs(src(i));
-        const double jn = jaclog(-ai, 50.0);
+        const double jn = jaclog(-ai, 50.0, lt);
         pm_hd += jn;
-        S.d(i) = (float)(jaclog(ai, 50.0) - jn);
+        S.d(i) = (float)(jaclog(ai, 50.0, lt) - jn);
     }
     place_by_rank(S, rank);
     S.d(-1) = 0.0f;
@@ -428,12 +438,12 @@ __device__ __forceinline__ bool line_setup_f32(const Src &src, const LDS &S,
 template <class P, class Src, class LDS>
 __device__ __forceinline__ bool line_setup_p(const Src &src, const LDS &S,
                                              uint32_t &hd, uint32_t &s0, double &pm_hd,
-                                             int32_t &ic)
+                                             int32_t &ic, const LogEntry *lt)
 {
     if constexpr (P::kF32)
-        return line_setup_f32(src, S, hd, s0, pm_hd, ic);
+        return line_setup_f32(src, S, hd, s0, pm_hd, ic, lt);
     else
-        return line_setup(src, S, hd, s0, pm_hd, ic, P::kLegacy);
+        return line_setup(src, S, hd, s0, pm_hd, ic, P::kLegacy, lt);
 }
 
 // Record a list entry and apply the stopping rule (:628-641, :674-686, getLConf :767-774).
@@ -850,7 +860,8 @@ __device__ __forceinline__ void line_search(const LDS &S, uint32_t hd,
 // 1e-9 in cubic_decoder.c:509-510); eps_log: floor inside the logs (EPSILON, or
 // 1e-30 in cubic_decoder.c:533).
 __device__ __forceinline__ double soft_app(const ListEntries &ls, int nl, double pnotin, int i,
-                                           double x, double eps, double eps_log)
+                                           double x, double eps, double eps_log,
+                                           const LogEntry *lt)
 {
     double pp1 = 1.0 / (1.0 + exp(x));
     double pp0 = 1.0 - pp1;
@@ -866,7 +877,7 @@ __device__ __forceinline__ double soft_app(const ListEntries &ls, int nl, double
         }
     p0 += pnotin * pp0;
     p1 += pnotin * pp1;
-    return log(fmax(p0, eps_log)) - log(fmax(p1, eps_log));
+    return log_pos(fmax(p0, eps_log), lt) - log_pos(fmax(p1, eps_log), lt);
 }
 
 } // namespace sg

@@ -86,12 +86,14 @@ __global__ __launch_bounds__(kPlainThreads) void plain_kernel(CubeJob job)
     __shared__ uint8_t s_hd[kCube];
     __shared__ uint32_t s_sum[kPlainThreads / 64], s_max[kPlainThreads / 64];
     __shared__ ExpEntry s_exp[kExpTableEntries];
+    __shared__ LogEntry s_log[kLogTableEntries];
 
     const int t = threadIdx.x;
     load_exp_table(s_exp, t);
+    load_log_table(s_log, t);
     __syncthreads();
     const LineLdsT<Rec, kPlainThreads> S(s_delta, s_hp, t);
-    const DecodeCfg cfg{job.tmax, job.thres, job.sched, job.L, s_exp, job.eps, job.eps_log};
+    const DecodeCfg cfg{job.tmax, job.thres, job.sched, job.L, s_exp, s_log, job.eps, job.eps_log};
     double *le = job.le_scratch + (size_t)blockIdx.x * kCube;
 
     for (size_t cube = blockIdx.x; cube < job.n_cw; cube += gridDim.x) {
@@ -109,7 +111,7 @@ __global__ __launch_bounds__(kPlainThreads) void plain_kernel(CubeJob job)
                 double pm_hd;
                 int32_t ic;
                 LineList ls;
-                if (line_setup_p<P>(src, S, hd, s0, pm_hd, ic)) {
+                if (line_setup_p<P>(src, S, hd, s0, pm_hd, ic, cfg.logtab)) {
                     line_search<FINITE_T, P>(S, hd, s0, pm_hd, ic, cfg, ls);
                 } else {
                     bad = true;
@@ -129,7 +131,7 @@ __global__ __launch_bounds__(kPlainThreads) void plain_kernel(CubeJob job)
                         la = xin;
                         lenew = 0.0;
                     } else {
-                        la = soft_app(ent, ls.nl, pnotin, i, xin, cfg.eps, cfg.eps_log);
+                        la = soft_app(ent, ls.nl, pnotin, i, xin, cfg.eps, cfg.eps_log, cfg.logtab);
                         lenew = la - xin;
                     }
                     le[p] = lenew;
@@ -201,9 +203,12 @@ __global__ __launch_bounds__(256) void lines_kernel(const double *llr, size_t n,
     __shared__ Rec s_delta[17 * 256];
     __shared__ uint16_t s_hp[17 * 256];
     __shared__ ExpEntry s_exp[kExpTableEntries];
+    __shared__ LogEntry s_log[kLogTableEntries];
     load_exp_table(s_exp, threadIdx.x);
+    load_log_table(s_log, threadIdx.x);
     __syncthreads();
     cfg.exptab = s_exp;
+    cfg.logtab = s_log;
     const size_t line = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (line >= n) return;
     const LineLdsT<Rec, 256> S(s_delta, s_hp, (int)threadIdx.x);
@@ -213,7 +218,7 @@ __global__ __launch_bounds__(256) void lines_kernel(const double *llr, size_t n,
     double pm_hd;
     int32_t ic;
     LineList ls;
-    const bool ok = line_setup_p<P>(src, S, hd, s0, pm_hd, ic);
+    const bool ok = line_setup_p<P>(src, S, hd, s0, pm_hd, ic, cfg.logtab);
     if (ok) {
         line_search<FINITE_T, P>(S, hd, s0, pm_hd, ic, cfg, ls);
     } else {
@@ -232,7 +237,7 @@ __global__ __launch_bounds__(256) void lines_kernel(const double *llr, size_t n,
             a = xi;
             e = 0.0;
         } else {
-            a = soft_app(ent, ls.nl, pnotin, i, xi, cfg.eps, cfg.eps_log);
+            a = soft_app(ent, ls.nl, pnotin, i, xi, cfg.eps, cfg.eps_log, cfg.logtab);
             e = a - xi;
         }
         if (la) la[line * 16 + i] = a;
@@ -337,7 +342,7 @@ hipError_t launch_lines(const double *llr, size_t n_lines, double *la, double *l
                         hipStream_t st)
 {
     if (n_lines == 0) return hipSuccess;
-    const DecodeCfg cfg{tmax, thres, sched, L, nullptr, eps, eps_log};
+    const DecodeCfg cfg{tmax, thres, sched, L, nullptr, nullptr, eps, eps_log};
     const dim3 g((unsigned)((n_lines + 255) / 256)), b(256);
     const bool finite = tmax_binds(tmax);
 #define SG_LINES(FT, PP)                                                                      \

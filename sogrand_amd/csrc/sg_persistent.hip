@@ -133,7 +133,8 @@ __device__ __forceinline__ uint32_t xor16(uint32_t v)
 // Four waves per SIMD asked for: the single-precision variant's records fit
 // four workgroups' LDS per CU and its registers fit 128 with a few bytes of
 // spill, which measures faster than three waves; the FP64 kernel's LDS allows
-// three workgroups per CU whatever the hint says.
+// three workgroups per CU whatever the hint says (and asking for three
+// instead measures 17 % slower: the search loop is scheduled differently).
 template <bool FINITE_T, int S, class P>
 __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))) void persistent_kernel(CubeJob job, uint32_t thresh,
                                                                  uint32_t step_len)
@@ -147,15 +148,17 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))
     __shared__ double s_stage_a[kWaves][4][2]; // per sub-round line: |x| of ranks 0 and 7
     __shared__ uint32_t s_geom[kWaves][S][2];  // per slot: cube, axis | iteration << 8
     __shared__ ExpEntry s_exp[kExpTableEntries];
+    __shared__ LogEntry s_log[kLogTableEntries];
 
     load_exp_table(s_exp, threadIdx.x);
+    load_log_table(s_log, threadIdx.x);
     __syncthreads(); // the only workgroup barrier: waves run independently from here
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint64_t lt_mask = (1ull << lane) - 1ull;
     Rec *rec_d = &s_d[wave][0];
     uint16_t *rec_h = &s_h[wave][0];
     const LineLdsT<Rec, 64> L(rec_d, rec_h, lane);
-    const DecodeCfg cfg{job.tmax, job.thres, job.sched, job.L, s_exp, job.eps, job.eps_log};
+    const DecodeCfg cfg{job.tmax, job.thres, job.sched, job.L, s_exp, s_log, job.eps, job.eps_log};
     const size_t wave_id = (size_t)blockIdx.x * kWaves + wave;
     uint32_t *hd_bits = &s_hd[wave][0][0];
     uint32_t *cnt = &s_cnt[wave][0][0];
@@ -351,7 +354,7 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))
                         lenew = 0.0;
                     } else {
                         la = soft_app(ent, ol.nl, fmax(ol.pnl, cfg.eps), i, xin, cfg.eps,
-                                      cfg.eps_log);
+                                      cfg.eps_log, cfg.logtab);
                         lenew = la - xin;
                     }
                     le[p] = lenew;
@@ -424,8 +427,8 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))
                         rank += (aj < key || (aj == key && j < i)) ? 1u : 0u;
                         if (!P::kF32) tie = tie || (aj == key && j != i);
                     }
-                    const double jn = jaclog(-ai, P::kJacLim);
-                    delta = jaclog(ai, P::kJacLim) - jn; // getPM's increment (:765)
+                    const double jn = jaclog(-ai, P::kJacLim, cfg.logtab);
+                    delta = jaclog(ai, P::kJacLim, cfg.logtab) - jn; // getPM's increment (:765)
                     stage_jn[lane] = jn;
                     // the two order statistics of IC do not depend on tie order
                     if (rank == 0) stage_a[2 * q] = (double)key;
@@ -474,7 +477,7 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))
                     } else {
                         if constexpr (!P::kF32) {
                             // records hold |x| in natural order
-                            if (line_special) pm_hd = setup_with_ties(L, P::kJacLim);
+                            if (line_special) pm_hd = setup_with_ties(L, P::kJacLim, cfg.logtab);
                         }
                         search_init<P, FINITE_T>(ss, ls, hd, s0, pm_hd, ic, cfg);
                         state = ss.done ? kDone : kSearch;
