@@ -17,24 +17,13 @@
 #include <stdint.h>
 
 #include "sg_common.h"
-#include "sg_exp_table.h"
+#include "sg_exp.h"
 #include "sg_log.h"
 
 #pragma clang fp contract(off)
 
 namespace sg {
 
-// One 16-byte entry of the LDS exp table. Entries 0..31 hold 2^(j/32) = a + b
-// (hi, lo); entries 32 and 33 hold the Taylor coefficients (1/6!, 1/5!) and
-// (1/4!, 1/3!) of exp(r) - 1, read at one address by every lane (a broadcast).
-// Reading the coefficients from LDS keeps them out of loop-invariant
-// registers, which the compiler otherwise pays for with a copy per step; as a
-// separate entry rather than a copy in every entry, the per-lane read at a
-// data-dependent index stays 16 bytes wide.
-struct ExpEntry {
-    double a, b;
-};
-constexpr int kExpTableEntries = 34;
 
 struct DecodeCfg {
     uint64_t tmax;          // 0 already mapped to 0x7fffffff by the host
@@ -51,9 +40,7 @@ struct DecodeCfg {
 // Fill the LDS exp table (first 34 threads) - the caller synchronises.
 __device__ __forceinline__ void load_exp_table(ExpEntry *lds, int tid)
 {
-    if (tid < 32) lds[tid] = ExpEntry{kExp2Table[tid].hi, kExp2Table[tid].lo};
-    if (tid == 32) lds[32] = ExpEntry{1.0 / 720.0, 1.0 / 120.0};
-    if (tid == 33) lds[33] = ExpEntry{1.0 / 24.0, 1.0 / 6.0};
+    if (tid < kExpTableEntries) lds[tid] = exp_table_entry(tid);
 }
 
 // Fill the LDS log table (first kLogTableEntries threads) - the caller
@@ -64,13 +51,8 @@ __device__ __forceinline__ void load_log_table(LogEntry *lds, int tid)
 }
 
 // exp(-pm) for pm >= 0, the probability weight of one guess (:626, :672, :712).
-// Tang-style: -pm = (32k + j) ln2/32 + r with |r| <= ln2/64, exp(r) - 1 = q by
-// its Taylor polynomial to r^6, and 2^(j/32) = hi + lo from the LDS table; the
-// result hi + (hi q + lo (1 + q)) scaled by 2^k. Within ~0.5 ulp (it differs
-// from the correctly rounded value on about 1 input in 200). Results below
-// 2^-1075 (pm beyond about 745.13) round to 0 in the final ldexp, as exp's do.
 // pm is capped at 746 (which also maps +inf there) to keep k in int range: at
-// the cap k >> 5 = -1077 and m < 2, so the result is 0 as well.
+// the cap k >> 5 = -1077 and m < 2, so the result is 0.
 __device__ __forceinline__ double exp_neg(double pm, const ExpEntry *tab)
 {
     // the cap as the instruction itself: as fmin the compiler first quiets a
@@ -78,23 +60,17 @@ __device__ __forceinline__ double exp_neg(double pm, const ExpEntry *tab)
     // infinite non-negative terms, never NaN
     double pc;
     asm("v_min_f64 %0, %1, %2" : "=v"(pc) : "v"(pm), "v"(746.0));
-    const double x = -pc;
-    const double kd = __builtin_rint(x * kExpInvLn2N);
-    const int k = (int)kd;
-    const ExpEntry t = tab[k & 31];
-    const ExpEntry c65 = tab[32], c43 = tab[33];
-    double r = __builtin_fma(-kd, kExpLn2NHi, x);
-    r = __builtin_fma(-kd, kExpLn2NLo, r);
-    double q = c65.a;
-    q = __builtin_fma(q, r, c65.b);
-    q = __builtin_fma(q, r, c43.a);
-    q = __builtin_fma(q, r, c43.b);
-    q = __builtin_fma(q, r, 0.5);
-    q = __builtin_fma(q, r, 1.0);
-    q = q * r;
-    const double s = __builtin_fma(t.b, q, t.b);
-    const double m = __builtin_fma(t.a, q, s) + t.a;
-    return __builtin_ldexp(m, k >> 5);
+    return exp_of(-pc, tab);
+}
+
+// exp(x) for any finite x: capped to [-746, 710], where the result is 0 or
+// overflows to +inf as exp's does (at 710, k >> 5 = 1024 and m > 1).
+__device__ __forceinline__ double exp_any(double x, const ExpEntry *tab)
+{
+    double xc;
+    asm("v_max_f64 %0, %1, %2" : "=v"(xc) : "v"(x), "v"(-746.0));
+    asm("v_min_f64 %0, %1, %2" : "=v"(xc) : "v"(xc), "v"(710.0));
+    return exp_of(xc, tab);
 }
 
 // Arithmetic of the per-guess path metrics.
@@ -168,11 +144,11 @@ __device__ __forceinline__ int lowest_bit(uint32_t x)
 }
 
 // JacLog (:762; cut at lim = 50, or 30 in cubic_decoder.c:699).
-__device__ __forceinline__ double jaclog(double x, double lim, const LogEntry *lt)
+__device__ __forceinline__ double jaclog(double x, double lim, const DecodeCfg &cfg)
 {
     if (x > lim) return x;
     if (x < -lim) return 0.0;
-    return log_pos(1.0 + exp(x), lt);
+    return log_pos(1.0 + exp_of(x, cfg.exptab), cfg.logtab);
 }
 
 
@@ -257,7 +233,7 @@ struct ListEntries {
 // (replayed by quicksort_lds), then PM_HD in natural order and the sorted
 // records. Returns PM_HD.
 template <class LDS>
-__device__ __forceinline__ double setup_with_ties(const LDS S, double jlim, const LogEntry *lt)
+__device__ __forceinline__ double setup_with_ties(const LDS S, double jlim, const DecodeCfg &cfg)
 {
 #pragma unroll 1
     for (int i = 0; i < 16; ++i) S.h(i) = (uint16_t)(hcol_of((unsigned)i) | (i << 8));
@@ -269,11 +245,11 @@ __device__ __forceinline__ double setup_with_ties(const LDS S, double jlim, cons
     double pm_hd = 0.0;
 #pragma unroll 1
     for (int i = 0; i < 16; ++i)
-        pm_hd += jaclog(-S.d((int)((slot_of >> (4 * i)) & 15u)), jlim, lt);
+        pm_hd += jaclog(-S.d((int)((slot_of >> (4 * i)) & 15u)), jlim, cfg);
 #pragma unroll 1
     for (int b = 0; b < 16; ++b) {
         const double ab = S.d(b);
-        S.d(b) = jaclog(ab, jlim, lt) - jaclog(-ab, jlim, lt);
+        S.d(b) = jaclog(ab, jlim, cfg) - jaclog(-ab, jlim, cfg);
     }
     return pm_hd;
 }
@@ -327,7 +303,7 @@ template <class LDS> __device__ __forceinline__ void place_by_rank(const LDS &S,
 template <class Src, class LDS>
 __device__ __forceinline__ bool line_setup(const Src &src, const LDS &S, uint32_t &hd,
                                            uint32_t &s0, double &pm_hd, int32_t &ic,
-                                           bool legacy, const LogEntry *lt)
+                                           bool legacy, const DecodeCfg &cfg)
 {
     const double jlim = legacy ? 30.0 : 50.0;
     bool finite = true;
@@ -371,13 +347,13 @@ __device__ __forceinline__ bool line_setup(const Src &src, const LDS &S, uint32_
 #pragma unroll 1
         for (int i = 0; i < 16; ++i) {
             const double ai = S.d(i);
-            const double jn = jaclog(-ai, jlim, lt);
+            const double jn = jaclog(-ai, jlim, cfg);
             pm_hd += jn;
-            S.d(i) = jaclog(ai, jlim, lt) - jn;
+            S.d(i) = jaclog(ai, jlim, cfg) - jn;
         }
         place_by_rank(S, rank);
     } else {
-        pm_hd = setup_with_ties(S, jlim, lt);
+        pm_hd = setup_with_ties(S, jlim, cfg);
     }
     S.d(-1) = 0.0; // null record read by exhausted masks in search_run
     S.h(-1) = 0;
@@ -391,7 +367,7 @@ __device__ __forceinline__ bool line_setup(const Src &src, const LDS &S, uint32_
 template <class Src, class LDS>
 __device__ __forceinline__ bool line_setup_f32(const Src &src, const LDS &S,
                                                uint32_t &hd, uint32_t &s0, double &pm_hd,
-                                               int32_t &ic, const LogEntry *lt)
+                                               int32_t &ic, const DecodeCfg &cfg)
 {
     bool finite = true;
     hd = 0;
@@ -424,9 +400,9 @@ __device__ __forceinline__ bool line_setup_f32(const Src &src, const LDS &S,
 #pragma unroll 1
     for (int i = 0; i < 16; ++i) {
         const double ai = fabs(src(i));
-        const double jn = jaclog(-ai, 50.0, lt);
+        const double jn = jaclog(-ai, 50.0, cfg);
         pm_hd += jn;
-        S.d(i) = (float)(jaclog(ai, 50.0, lt) - jn);
+        S.d(i) = (float)(jaclog(ai, 50.0, cfg) - jn);
     }
     place_by_rank(S, rank);
     S.d(-1) = 0.0f;
@@ -438,12 +414,12 @@ __device__ __forceinline__ bool line_setup_f32(const Src &src, const LDS &S,
 template <class P, class Src, class LDS>
 __device__ __forceinline__ bool line_setup_p(const Src &src, const LDS &S,
                                              uint32_t &hd, uint32_t &s0, double &pm_hd,
-                                             int32_t &ic, const LogEntry *lt)
+                                             int32_t &ic, const DecodeCfg &cfg)
 {
     if constexpr (P::kF32)
-        return line_setup_f32(src, S, hd, s0, pm_hd, ic, lt);
+        return line_setup_f32(src, S, hd, s0, pm_hd, ic, cfg);
     else
-        return line_setup(src, S, hd, s0, pm_hd, ic, P::kLegacy, lt);
+        return line_setup(src, S, hd, s0, pm_hd, ic, P::kLegacy, cfg);
 }
 
 // Record a list entry and apply the stopping rule (:628-641, :674-686, getLConf :767-774).
@@ -861,9 +837,9 @@ __device__ __forceinline__ void line_search(const LDS &S, uint32_t hd,
 // 1e-30 in cubic_decoder.c:533).
 __device__ __forceinline__ double soft_app(const ListEntries &ls, int nl, double pnotin, int i,
                                            double x, double eps, double eps_log,
-                                           const LogEntry *lt)
+                                           const DecodeCfg &cfg)
 {
-    double pp1 = 1.0 / (1.0 + exp(x));
+    double pp1 = 1.0 / (1.0 + exp_any(x, cfg.exptab));
     double pp0 = 1.0 - pp1;
     pp1 = fmin(fmax(pp1, eps), 1.0 - eps);
     pp0 = fmin(fmax(pp0, eps), 1.0 - eps);
@@ -877,7 +853,7 @@ __device__ __forceinline__ double soft_app(const ListEntries &ls, int nl, double
         }
     p0 += pnotin * pp0;
     p1 += pnotin * pp1;
-    return log_pos(fmax(p0, eps_log), lt) - log_pos(fmax(p1, eps_log), lt);
+    return log_pos(fmax(p0, eps_log), cfg.logtab) - log_pos(fmax(p1, eps_log), cfg.logtab);
 }
 
 } // namespace sg

@@ -111,7 +111,7 @@ __global__ __launch_bounds__(kPlainThreads) void plain_kernel(CubeJob job)
                 double pm_hd;
                 int32_t ic;
                 LineList ls;
-                if (line_setup_p<P>(src, S, hd, s0, pm_hd, ic, cfg.logtab)) {
+                if (line_setup_p<P>(src, S, hd, s0, pm_hd, ic, cfg)) {
                     line_search<FINITE_T, P>(S, hd, s0, pm_hd, ic, cfg, ls);
                 } else {
                     bad = true;
@@ -131,7 +131,7 @@ __global__ __launch_bounds__(kPlainThreads) void plain_kernel(CubeJob job)
                         la = xin;
                         lenew = 0.0;
                     } else {
-                        la = soft_app(ent, ls.nl, pnotin, i, xin, cfg.eps, cfg.eps_log, cfg.logtab);
+                        la = soft_app(ent, ls.nl, pnotin, i, xin, cfg.eps, cfg.eps_log, cfg);
                         lenew = la - xin;
                     }
                     le[p] = lenew;
@@ -218,7 +218,7 @@ __global__ __launch_bounds__(256) void lines_kernel(const double *llr, size_t n,
     double pm_hd;
     int32_t ic;
     LineList ls;
-    const bool ok = line_setup_p<P>(src, S, hd, s0, pm_hd, ic, cfg.logtab);
+    const bool ok = line_setup_p<P>(src, S, hd, s0, pm_hd, ic, cfg);
     if (ok) {
         line_search<FINITE_T, P>(S, hd, s0, pm_hd, ic, cfg, ls);
     } else {
@@ -237,7 +237,7 @@ __global__ __launch_bounds__(256) void lines_kernel(const double *llr, size_t n,
             a = xi;
             e = 0.0;
         } else {
-            a = soft_app(ent, ls.nl, pnotin, i, xi, cfg.eps, cfg.eps_log, cfg.logtab);
+            a = soft_app(ent, ls.nl, pnotin, i, xi, cfg.eps, cfg.eps_log, cfg);
             e = a - xi;
         }
         if (la) la[line * 16 + i] = a;

@@ -354,7 +354,7 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))
                         lenew = 0.0;
                     } else {
                         la = soft_app(ent, ol.nl, fmax(ol.pnl, cfg.eps), i, xin, cfg.eps,
-                                      cfg.eps_log, cfg.logtab);
+                                      cfg.eps_log, cfg);
                         lenew = la - xin;
                     }
                     le[p] = lenew;
@@ -427,8 +427,8 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))
                         rank += (aj < key || (aj == key && j < i)) ? 1u : 0u;
                         if (!P::kF32) tie = tie || (aj == key && j != i);
                     }
-                    const double jn = jaclog(-ai, P::kJacLim, cfg.logtab);
-                    delta = jaclog(ai, P::kJacLim, cfg.logtab) - jn; // getPM's increment (:765)
+                    const double jn = jaclog(-ai, P::kJacLim, cfg);
+                    delta = jaclog(ai, P::kJacLim, cfg) - jn; // getPM's increment (:765)
                     stage_jn[lane] = jn;
                     // the two order statistics of IC do not depend on tie order
                     if (rank == 0) stage_a[2 * q] = (double)key;
@@ -477,7 +477,7 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))
                     } else {
                         if constexpr (!P::kF32) {
                             // records hold |x| in natural order
-                            if (line_special) pm_hd = setup_with_ties(L, P::kJacLim, cfg.logtab);
+                            if (line_special) pm_hd = setup_with_ties(L, P::kJacLim, cfg);
                         }
                         search_init<P, FINITE_T>(ss, ls, hd, s0, pm_hd, ic, cfg);
                         state = ss.done ? kDone : kSearch;

@@ -1,7 +1,8 @@
-// Host build of the decoder's logarithm (sogrand_amd/csrc/sg_log.h, the same
-// source the kernels compile) measured against the C library's log and
-// against logl rounded to double. Prints one line of counts:
-//   n mismatch_vs_libc mismatch_vs_logl libc_vs_logl max_err_ulp_x1e4 special_fail
+// Host builds of the decoder's logarithm and exponential (sogrand_amd/csrc/
+// sg_log.h and sg_exp.h, the same sources the kernels compile) measured against
+// the C library's log and exp and against logl / expl rounded to double.
+// Prints two lines of counts, "log ..." and "exp ...":
+//   n mismatch_vs_libc mismatch_vs_long_double libc_vs_long_double max_err_ulp_x1e4 special_fail
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -9,6 +10,7 @@
 #include <cstdlib>
 #include <random>
 
+#include "sg_exp.h"
 #include "sg_log.h"
 
 int main(int argc, char **argv)
@@ -45,7 +47,37 @@ int main(int argc, char **argv)
     special += sg::log_pos(1.0, sg::kLogTable) != 0.0;
     for (int k = -1000; k <= 1000; k += 37)
         special += sg::log_pos(ldexp(1.0, k), sg::kLogTable) != (double)(k * logl(2.0L));
-    printf("%ld %ld %ld %ld %ld %ld\n", n, vs_libc, vs_logl, libc_vs_logl, (long)(max_err * 1e4),
-           special);
+    printf("log %ld %ld %ld %ld %ld %ld\n", n, vs_libc, vs_logl, libc_vs_logl,
+           (long)(max_err * 1e4), special);
+
+    sg::ExpEntry tab[sg::kExpTableEntries];
+    for (int i = 0; i < sg::kExpTableEntries; ++i) tab[i] = sg::exp_table_entry(i);
+    long e_libc = 0, e_ld = 0, libc_ld = 0;
+    double e_max = 0.0;
+    for (long j = 0; j < n; ++j) {
+        double x;
+        switch (j % 3) {
+        case 0: x = (u(g) - 0.5) * 100.0; break;  // JacLog's exp(+-a), |a| <= 50
+        case 1: x = -700.0 * u(g); break;         // guess weights exp(-pm)
+        default: x = (u(g) - 0.5) * 1400.0; break; // soft output's exp(x)
+        }
+        const double a = sg::exp_of(x, tab);
+        const long double E = expl((long double)x);
+        const double b = exp(x), cr = (double)E;
+        e_libc += a != b;
+        e_ld += a != cr;
+        libc_ld += b != cr;
+        const long double ulp = (long double)nextafter(cr, INFINITY) - (long double)cr;
+        const double e = (double)(fabsl((long double)a - E) / ulp);
+        if (e > e_max) e_max = e;
+    }
+    long e_special = 0;
+    e_special += sg::exp_of(0.0, tab) != 1.0;
+    e_special += sg::exp_of(-746.0, tab) != 0.0;
+    e_special += sg::exp_of(710.0, tab) != INFINITY;
+    for (int k = -1000; k <= 1000; k += 37) // x = k ln2 rounded: within 2^-52 |k| of 2^k
+        e_special += fabs(sg::exp_of((double)(k * logl(2.0L)), tab) / ldexp(1.0, k) - 1.0) > 0x1p-40;
+    printf("exp %ld %ld %ld %ld %ld %ld\n", n, e_libc, e_ld, libc_ld, (long)(e_max * 1e4),
+           e_special);
     return 0;
 }
