@@ -218,8 +218,14 @@ int sogrand_ctx_create(sogrand_ctx **out, int device, const sogrand_params *para
     uint16_t *masks = reinterpret_cast<uint16_t *>(buf.data());
     uint32_t *words = reinterpret_cast<uint32_t *>(buf.data() + sg::kSchedPosOffsetBytes);
     sg::build_all_schedule_tables(masks);
-    for (size_t i = 0; i < (size_t)sg::kSchedTables * sg::kSchedStride; ++i)
-        words[i] = sg::pos_word(masks[i]);
+    for (int t = 0; t < sg::kSchedTables; ++t) {
+        const size_t o = (size_t)t * sg::kSchedStride;
+        for (size_t j = 0; j < (size_t)sg::kSchedStride; ++j) {
+            const bool level = j == 0 || sg::pattern_wt(t, masks[o + j]) !=
+                                             sg::pattern_wt(t, masks[o + j - 1]);
+            words[o + j] = sg::pos_word(masks[o + j], level);
+        }
+    }
     if (hipMalloc(&c->d_sched, buf.size()) != hipSuccess) {
         sogrand_ctx_destroy(c);
         return SOGRAND_ENOMEM;

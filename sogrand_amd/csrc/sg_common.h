@@ -120,31 +120,53 @@ static_assert(kSortNet.n == 63, "odd-even merge sort of 16 keys has 63 comparato
 // ---------------------------------------------------------------------------
 constexpr int kSchedLen = 65535;    // nonzero patterns
 constexpr int kSchedStride = 65536; // table stride (entry 65535 = 0, end marker)
-constexpr int kSchedTables = 56;    // IC = 0..54 and one generic table for IC >= 55
-constexpr int kSchedGenericIc = 55;
+constexpr int kSchedTables = 57;    // IC = 0..55 and one generic table for IC >= 56
+constexpr int kSchedGenericIc = 56;
 
-// The same schedules as position words, one u32 per entry: bits 5i .. 5i + 4
-// hold b + 1 for the i-th lowest set bit b of the mask (i = 0..5), 0 where the
-// mask has fewer bits, and bit 30 is set when it has more than six (the rest
-// are then read from the mask). The end marker (mask 0) is the word 0. A field
-// is a record row of the line's LDS records (row 0 the null record), so a
-// pattern's positions come out with one bit-field extract each.
+// The same schedules as position words, one u32 per entry: bits 1 + 5i ..
+// 5 + 5i hold b + 1 for the i-th lowest set bit b of the mask (i = 0..5), 0
+// where the mask has fewer bits; bit 31 is set when it has more than six (the
+// rest are then read from the mask) and bit 0 when the entry starts a new
+// weight level of its table (the reference's budget guard runs there, :652).
+// The end marker (mask 0) is the word 0. A field is a record row of the line's
+// LDS records (row 0 the null record), so a pattern's positions come out with
+// one bit-field extract each.
+//
+// Weight levels: wt = IC w + S. For IC >= 56 two entries have equal wt only if
+// they have equal (w, S) (IC (w1 - w2) = S2 - S1 needs IC <= 55), so one
+// generic table serves every IC >= 56, order and levels alike.
 constexpr uint32_t kPosFields = 6;
-constexpr uint32_t kPosMore = 1u << 30;
-inline constexpr uint32_t pos_word(uint32_t m)
+constexpr uint32_t kPosLevel = 1u;
+constexpr uint32_t kPosMore = 1u << 31;
+inline constexpr uint32_t pos_word(uint32_t m, bool level_start)
 {
     uint32_t w = 0;
     for (uint32_t i = 0; i < kPosFields && m; ++i) {
         uint32_t b = 0;
         while (!((m >> b) & 1u)) ++b;
-        w |= (b + 1u) << (5u * i);
+        w |= (b + 1u) << (1u + 5u * i);
         m &= m - 1u;
     }
-    return m ? (w | kPosMore) : w;
+    if (m) w |= kPosMore;
+    return w && level_start ? (w | kPosLevel) : w;
 }
-static_assert(pos_word(0x1u) == 1u && pos_word(0x8001u) == (1u | (16u << 5)), "position word");
-static_assert(pos_word(0x7fu) == (kPosMore | 1u | 2u << 5 | 3u << 10 | 4u << 15 | 5u << 20 | 6u << 25),
+static_assert(pos_word(0x1u, false) == 2u && pos_word(0x8001u, true) == (1u | 2u | (16u << 6)),
+              "position word");
+static_assert(pos_word(0x7fu, false) ==
+                  (kPosMore | 1u << 1 | 2u << 6 | 3u << 11 | 4u << 16 | 5u << 21 | 6u << 26),
               "position word overflow");
+static_assert(pos_word(0u, true) == 0u, "end marker");
+// Weight level of a mask under intercept ic (getPM's landslide weight, :646-653).
+inline constexpr int64_t pattern_wt(int64_t ic, uint32_t m)
+{
+    int64_t w = 0, sum = 0;
+    for (uint32_t b = 0; b < 16; ++b)
+        if ((m >> b) & 1u) {
+            w += 1;
+            sum += b + 1;
+        }
+    return ic * w + sum;
+}
 // Device layout of the schedule buffer: the mask tables, 8 zero masks of
 // padding, then the position-word tables and 8 zero words of padding (the
 // search loops prefetch one chunk past an end marker).

@@ -488,16 +488,6 @@ __device__ __forceinline__ void list_entries(const LineList &ls, const LDS &S, u
     }
 }
 
-// Sum of (b + 1) over the set bits b of a 16-bit mask: the landslide weight
-// of the pattern without its intercept part (wt = IC w + S, :646-653). Bit k of
-// the position b is counted by popcount(m & the mask of positions with bit k).
-__device__ __forceinline__ int position_sum(uint32_t m)
-{
-    return __builtin_popcount(m) + __builtin_popcount(m & 0xAAAAu) +
-           2 * __builtin_popcount(m & 0xCCCCu) + 4 * __builtin_popcount(m & 0xF0F0u) +
-           8 * __builtin_popcount(m & 0xFF00u);
-}
-
 // State of one line's guessing loop (sogrand_main_logic :624-744), so that
 // the loop can be run in bounded steps.
 struct SearchState {
@@ -505,11 +495,9 @@ struct SearchState {
                       // hard decision and t patterns)
     double pnot;      // P_notGuess (:614, :626, :672, :712)
     bool done;
-    const uint4 *tp;  // schedule table of the line's intercept: masks, or
-                      // position words in the unbudgeted search
-    uint4 nxt;        // the next chunk of 8 masks (prefetched), or the first
-    uint4 nxt2;       // and second halves of the next 8 position words
-    int64_t cur_wt;   // weight level of the last pattern (finite Tmax only)
+    const uint4 *tp;  // position-word table of the line's intercept
+    uint4 nxt;        // the next chunk of 8 position words (prefetched):
+    uint4 nxt2;       // first and second halves
 };
 
 // The position-word tables, at a fixed offset behind the mask tables in the
@@ -520,9 +508,8 @@ __device__ __forceinline__ const uint32_t *pos_tables(const uint16_t *sched)
                                               kSchedPosOffsetBytes);
 }
 
-// Hard-decision guess (:624-641) and schedule selection (position words for
-// the unbudgeted search, masks under a guess budget).
-template <class P, bool FINITE_T>
+// Hard-decision guess (:624-641) and schedule selection.
+template <class P>
 __device__ __forceinline__ void search_init(SearchState &ss, LineList &ls, uint32_t hd, uint32_t s0,
                                             double pm_hd, int32_t ic, const DecodeCfg &cfg)
 {
@@ -542,16 +529,10 @@ __device__ __forceinline__ void search_init(SearchState &ss, LineList &ls, uint3
     ss.done = false;
     if (s0 == 0) ss.done = list_hit(ls, e_hd, 0u, ss.pnot, cfg);
     const int tbl = ic < kSchedGenericIc ? ic : kSchedGenericIc;
-    if constexpr (FINITE_T) {
-        ss.tp = reinterpret_cast<const uint4 *>(cfg.sched + (size_t)tbl * kSchedStride);
-        ss.nxt = ss.tp[0];
-    } else {
-        ss.tp = reinterpret_cast<const uint4 *>(pos_tables(cfg.sched) + (size_t)tbl * kSchedStride);
-        ss.nxt = ss.tp[0];
-        ss.nxt2 = ss.tp[1];
-    }
+    ss.tp = reinterpret_cast<const uint4 *>(pos_tables(cfg.sched) + (size_t)tbl * kSchedStride);
+    ss.nxt = ss.tp[0];
+    ss.nxt2 = ss.tp[1];
     ss.t = 0;
-    ss.cur_wt = -1;
 }
 
 // Accounting of a pair of guesses at schedule positions t and t + 1 from their
@@ -602,84 +583,6 @@ __device__ __forceinline__ void pair_account(SearchState &ss, LineList &ls, cons
     ss.done = stop;
 }
 
-// Two consecutive guesses of the schedule (:652-744): the patterns of `pair`
-// (two 16-bit masks) at schedule positions t and t + 1. Each lane carries two
-// independent path-metric and exp chains; the guesses are still accounted
-// strictly in schedule order (the second is dropped if the first ends the
-// search).
-//
-// In the earlier decoder (P::kLegacy) P_notGuess never changes, so a pattern's
-// metric matters only if it is a codeword or, under a guess budget, the last
-// guess (whose weight is the P_neg a budget stop leaves, cubic_decoder.c:638).
-// Without a budget the loop forms syndromes alone and a hit's metric is summed
-// from its mask afterwards, with the same additions.
-template <bool FINITE_T, class P, class LDS>
-__device__ __forceinline__ void search_pair(SearchState &ss, LineList &ls, const LDS &S,
-                                            uint32_t s0, double pm_hd, int32_t ic,
-                                            const DecodeCfg &cfg, uint32_t pair, uint32_t t)
-{
-    using Rec = typename P::Rec;
-    static_assert(sizeof(Rec) == sizeof(typename LDS::Rec), "record type");
-    constexpr bool kPmLoop = !(P::kLegacy && !FINITE_T);
-    // Masks 0 .. 65534 of a schedule are nonzero and mask 65535 is the zero
-    // pad, so m1 is never 0: the pair (65534, 65535) ends the search through
-    // take2 = false (all patterns tested, wt > wt_max).
-    const uint32_t m1 = pair & 0xffffu, m2 = pair >> 16;
-    bool take2 = m2 != 0;
-    if (FINITE_T) { // loop guard once per weight level (:652)
-        const int64_t wt1 = (int64_t)ic * __builtin_popcount(m1) + position_sum(m1);
-        const int64_t wt2 = (int64_t)ic * __builtin_popcount(m2) + position_sum(m2);
-        if (wt1 != ss.cur_wt) {
-            if (!((uint64_t)ss.t + 1u < cfg.tmax)) { // T < Tmax, T = t + 1
-                ss.done = true;
-                return;
-            }
-            ss.cur_wt = wt1;
-        }
-        if (take2 && wt2 != wt1 && !((uint64_t)ss.t + 2u < cfg.tmax)) take2 = false;
-        else if (take2) ss.cur_wt = wt2;
-    }
-    // path metrics (ascending sorted position, as getPM adds them) and
-    // syndromes, two set bits of each pattern per iteration; an exhausted mask
-    // gives position -1, the null record (delta +0.0, H column 0), which leaves
-    // both sums unchanged, so no lane needs a select or a branch
-    // (the first round reads the hard decision's metric and syndrome directly)
-    const Rec pm_base = (Rec)pm_hd;
-    Rec pm1 = 0, pm2 = 0;
-    uint32_t syn1, syn2;
-    uint32_t r1 = m1, r2 = take2 ? m2 : 0u;
-    {
-        const uint32_t q1 = r1 & (r1 - 1u), q2 = r2 & (r2 - 1u);
-        const int b1 = lowest_bit(r1), c1 = lowest_bit(q1);
-        const int b2 = lowest_bit(r2), c2 = lowest_bit(q2);
-        syn1 = s0 ^ (uint32_t)S.h(b1) ^ (uint32_t)S.h(c1);
-        syn2 = s0 ^ (uint32_t)S.h(b2) ^ (uint32_t)S.h(c2);
-        if constexpr (kPmLoop) {
-            const Rec d1 = S.d(b1), e1 = S.d(c1), d2 = S.d(b2), e2 = S.d(c2);
-            pm1 = (pm_base + d1) + e1;
-            pm2 = (pm_base + d2) + e2;
-        }
-        r1 = q1 & (q1 - 1u);
-        r2 = q2 & (q2 - 1u);
-    }
-    while (r1 | r2) {
-        const uint32_t q1 = r1 & (r1 - 1u), q2 = r2 & (r2 - 1u);
-        const int b1 = lowest_bit(r1), c1 = lowest_bit(q1);
-        const int b2 = lowest_bit(r2), c2 = lowest_bit(q2);
-        syn1 ^= (uint32_t)S.h(b1) ^ (uint32_t)S.h(c1);
-        syn2 ^= (uint32_t)S.h(b2) ^ (uint32_t)S.h(c2);
-        if constexpr (kPmLoop) {
-            const Rec d1 = S.d(b1), e1 = S.d(c1), d2 = S.d(b2), e2 = S.d(c2);
-            pm1 = (pm1 + d1) + e1;
-            pm2 = (pm2 + d2) + e2;
-        }
-        r1 = q1 & (q1 - 1u);
-        r2 = q2 & (q2 - 1u);
-    }
-    pair_account<P, kPmLoop>(ss, ls, S, pm_hd, cfg, pm1, pm2, syn1, syn2, take2, t,
-                             [&](int j) { return j == 0 ? m1 : m2; });
-}
-
 template <int I> struct IntC {
     static constexpr int value = I;
 };
@@ -690,26 +593,47 @@ template <int I> struct IntC {
 template <int I> __device__ __forceinline__ uint32_t pos_field(uint32_t w)
 {
     uint32_t f;
-    asm("v_bfe_u32 %0, %1, %2, 5" : "=v"(f) : "v"(w), "i"(5 * I));
+    asm("v_bfe_u32 %0, %1, %2, 5" : "=v"(f) : "v"(w), "i"(1 + 5 * I));
     return f;
 }
 
-// The same pair of guesses from position words (the unbudgeted search): the
-// fields of both words are read two at a time; the rounds past the first take
-// a wave-uniform branch, so a round runs only if some lane's pattern has that
-// many set bits, and a lane whose pattern has fewer reads the null record
-// (row 0), which leaves both sums unchanged. Bits past the sixth (rare) come
-// from the mask. The sums run in ascending sorted position, as in search_pair.
-template <class P, class LDS>
-__device__ __forceinline__ void search_pair_pos(SearchState &ss, LineList &ls, const LDS &S,
-                                                uint32_t s0, double pm_hd, int32_t ic,
-                                                const DecodeCfg &cfg, uint32_t w1, uint32_t w2,
-                                                uint32_t t)
+// Two consecutive guesses of the schedule (:652-744): the patterns of the
+// position words w1 and w2 at schedule positions t and t + 1. Each lane
+// carries two independent path-metric and exp chains; the guesses are still
+// accounted strictly in schedule order (the second is dropped if the first
+// ends the search). The fields of both words are read two at a time; the
+// rounds past the first take a wave-uniform branch, so a round runs only if
+// some lane's pattern has that many set bits, and a lane whose pattern has
+// fewer reads the null record (row 0), which leaves both sums unchanged. Bits
+// past the sixth (rare) come from the mask. The path metrics add the
+// increments in ascending sorted position, as getPM does (:765).
+//
+// Under a guess budget the guard T < Tmax runs once per weight level (:652),
+// before the first pattern of each level (the words' level bit), with T = t + 1.
+// In the earlier decoder (P::kLegacy) P_notGuess never changes, so a pattern's
+// metric matters only if it is a codeword or, under a budget, the last guess
+// (whose weight is the P_neg a budget stop leaves, cubic_decoder.c:638).
+// Without a budget the loop forms syndromes alone and a hit's metric is summed
+// from its mask afterwards, with the same additions.
+template <bool FINITE_T, class P, class LDS>
+__device__ __forceinline__ void search_pair(SearchState &ss, LineList &ls, const LDS &S,
+                                            uint32_t s0, double pm_hd, int32_t ic,
+                                            const DecodeCfg &cfg, uint32_t w1, uint32_t w2,
+                                            uint32_t t)
 {
     using Rec = typename P::Rec;
-    constexpr bool kPmLoop = !P::kLegacy;
+    static_assert(sizeof(Rec) == sizeof(typename LDS::Rec), "record type");
+    constexpr bool kPmLoop = !(P::kLegacy && !FINITE_T);
     // the end marker is the word 0; every other word has a nonzero first field
-    const bool take2 = w2 != 0;
+    bool take2 = w2 != 0;
+    if (FINITE_T) {
+        if ((w1 & kPosLevel) && !((uint64_t)t + 1u < cfg.tmax)) {
+            ss.done = true;
+            return;
+        }
+        if (take2 && (w2 & kPosLevel) && !((uint64_t)t + 2u < cfg.tmax)) take2 = false;
+    }
+    if (!take2) w2 = 0;
     const Rec pm_base = (Rec)pm_hd;
     Rec pm1 = 0, pm2 = 0;
     uint32_t syn1, syn2;
@@ -734,9 +658,10 @@ __device__ __forceinline__ void search_pair_pos(SearchState &ss, LineList &ls, c
             pm2 = (pm2 + S.dr(a2)) + S.dr(b2);
         }
     };
-    if (__ballot(((w1 | w2) >> 10) != 0u) != 0ull) {
+    // fields 2.. (and the overflow bit) sit above bit 10, fields 4.. above bit 20
+    if (__ballot(((w1 | w2) >> 11) != 0u) != 0ull) {
         round(IntC<2>{});
-        if (__ballot(((w1 | w2) >> 20) != 0u) != 0ull) round(IntC<4>{});
+        if (__ballot(((w1 | w2) >> 21) != 0u) != 0ull) round(IntC<4>{});
     }
     const int tbl = ic < kSchedGenericIc ? ic : kSchedGenericIc;
     const uint16_t *masks = cfg.sched + (size_t)tbl * kSchedStride;
@@ -758,45 +683,18 @@ __device__ __forceinline__ void search_pair_pos(SearchState &ss, LineList &ls, c
                              [&](int j) { return (uint32_t)masks[t + (uint32_t)j]; });
 }
 
-// Further guesses of the schedule, one chunk of 8 entries (four pairs: 16
-// bytes of masks, or 32 bytes of position words without a guess budget) per
-// iteration, up to about max_steps guesses. A search starts at a chunk
-// boundary and a lane that stops mid-chunk is done, so every searching lane
-// enters an iteration at a chunk boundary; the chunk was loaded one iteration
-// ahead. The loop runs an iteration count common to the wave and leaves when
-// no lane is still searching; a finished lane idles inside it, which keeps the
-// wave's control flow to one mask.
+// Further guesses of the schedule, one chunk of 8 position words (four pairs,
+// 32 bytes) per iteration, up to about max_steps guesses. A search starts at a
+// chunk boundary and a lane that stops mid-chunk is done, so every searching
+// lane enters an iteration at a chunk boundary; the chunk was loaded one
+// iteration ahead. The loop runs an iteration count common to the wave and
+// leaves when no lane is still searching; a finished lane idles inside it,
+// which keeps the wave's control flow to one mask.
 template <bool FINITE_T, class P, class LDS>
 __device__ __forceinline__ void search_run(SearchState &ss, LineList &ls, const LDS &S, uint32_t hd,
                                            uint32_t s0, double pm_hd, int32_t ic,
                                            const DecodeCfg &cfg, uint32_t max_steps)
 {
-    if constexpr (!FINITE_T) {
-        // position words: a chunk of 8 is two 16-byte halves
-#pragma unroll 1
-        for (uint32_t step = 0; step < max_steps; step += 8) {
-            if (__ballot(!ss.done) == 0ull) break;
-            if (ss.done) continue;
-            const uint32_t t = ss.t; // a multiple of 8 here
-            uint4 c, d;
-            asm volatile("v_mov_b32 %0, %1" : "=v"(c.x) : "v"(ss.nxt.x) : "memory");
-            asm volatile("v_mov_b32 %0, %1" : "=v"(c.y) : "v"(ss.nxt.y) : "memory");
-            asm volatile("v_mov_b32 %0, %1" : "=v"(c.z) : "v"(ss.nxt.z) : "memory");
-            asm volatile("v_mov_b32 %0, %1" : "=v"(c.w) : "v"(ss.nxt.w) : "memory");
-            asm volatile("v_mov_b32 %0, %1" : "=v"(d.x) : "v"(ss.nxt2.x) : "memory");
-            asm volatile("v_mov_b32 %0, %1" : "=v"(d.y) : "v"(ss.nxt2.y) : "memory");
-            asm volatile("v_mov_b32 %0, %1" : "=v"(d.z) : "v"(ss.nxt2.z) : "memory");
-            asm volatile("v_mov_b32 %0, %1" : "=v"(d.w) : "v"(ss.nxt2.w) : "memory");
-            const uint4 *np = ss.tp + (t >> 2) + 2; // tables carry one chunk of padding
-            ss.nxt = np[0];
-            ss.nxt2 = np[1];
-            search_pair_pos<P>(ss, ls, S, s0, pm_hd, ic, cfg, c.x, c.y, t);
-            if (!ss.done) search_pair_pos<P>(ss, ls, S, s0, pm_hd, ic, cfg, c.z, c.w, t + 2u);
-            if (!ss.done) search_pair_pos<P>(ss, ls, S, s0, pm_hd, ic, cfg, d.x, d.y, t + 4u);
-            if (!ss.done) search_pair_pos<P>(ss, ls, S, s0, pm_hd, ic, cfg, d.z, d.w, t + 6u);
-        }
-        return;
-    }
 #pragma unroll 1
     for (uint32_t step = 0; step < max_steps; step += 8) {
         if (__ballot(!ss.done) == 0ull) break;
@@ -807,16 +705,22 @@ __device__ __forceinline__ void search_run(SearchState &ss, LineList &ls, const 
         // the end of the block, after the load, so the load lands in a
         // temporary and is moved into the prefetch registers at once, which
         // waits for it and defeats the prefetch.
-        uint4 c;
+        uint4 c, d;
         asm volatile("v_mov_b32 %0, %1" : "=v"(c.x) : "v"(ss.nxt.x) : "memory");
         asm volatile("v_mov_b32 %0, %1" : "=v"(c.y) : "v"(ss.nxt.y) : "memory");
         asm volatile("v_mov_b32 %0, %1" : "=v"(c.z) : "v"(ss.nxt.z) : "memory");
         asm volatile("v_mov_b32 %0, %1" : "=v"(c.w) : "v"(ss.nxt.w) : "memory");
-        ss.nxt = ss.tp[(t >> 3) + 1]; // the table buffer carries one chunk of padding
-        search_pair<FINITE_T, P>(ss, ls, S, s0, pm_hd, ic, cfg, c.x, t);
-        if (!ss.done) search_pair<FINITE_T, P>(ss, ls, S, s0, pm_hd, ic, cfg, c.y, t + 2u);
-        if (!ss.done) search_pair<FINITE_T, P>(ss, ls, S, s0, pm_hd, ic, cfg, c.z, t + 4u);
-        if (!ss.done) search_pair<FINITE_T, P>(ss, ls, S, s0, pm_hd, ic, cfg, c.w, t + 6u);
+        asm volatile("v_mov_b32 %0, %1" : "=v"(d.x) : "v"(ss.nxt2.x) : "memory");
+        asm volatile("v_mov_b32 %0, %1" : "=v"(d.y) : "v"(ss.nxt2.y) : "memory");
+        asm volatile("v_mov_b32 %0, %1" : "=v"(d.z) : "v"(ss.nxt2.z) : "memory");
+        asm volatile("v_mov_b32 %0, %1" : "=v"(d.w) : "v"(ss.nxt2.w) : "memory");
+        const uint4 *np = ss.tp + (t >> 2) + 2; // tables carry one chunk of padding
+        ss.nxt = np[0];
+        ss.nxt2 = np[1];
+        search_pair<FINITE_T, P>(ss, ls, S, s0, pm_hd, ic, cfg, c.x, c.y, t);
+        if (!ss.done) search_pair<FINITE_T, P>(ss, ls, S, s0, pm_hd, ic, cfg, c.z, c.w, t + 2u);
+        if (!ss.done) search_pair<FINITE_T, P>(ss, ls, S, s0, pm_hd, ic, cfg, d.x, d.y, t + 4u);
+        if (!ss.done) search_pair<FINITE_T, P>(ss, ls, S, s0, pm_hd, ic, cfg, d.z, d.w, t + 6u);
     }
 }
 
@@ -826,7 +730,7 @@ __device__ __forceinline__ void line_search(const LDS &S, uint32_t hd,
                                             const DecodeCfg &cfg, LineList &ls)
 {
     SearchState ss;
-    search_init<P, FINITE_T>(ss, ls, hd, s0, pm_hd, ic, cfg);
+    search_init<P>(ss, ls, hd, s0, pm_hd, ic, cfg);
     search_run<FINITE_T, P>(ss, ls, S, hd, s0, pm_hd, ic, cfg, 0xffffffffu);
     ls.T = ss.t + 1u;
 }

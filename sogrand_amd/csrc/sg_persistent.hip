@@ -479,7 +479,7 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))
                             // records hold |x| in natural order
                             if (line_special) pm_hd = setup_with_ties(L, P::kJacLim, cfg);
                         }
-                        search_init<P, FINITE_T>(ss, ls, hd, s0, pm_hd, ic, cfg);
+                        search_init<P>(ss, ls, hd, s0, pm_hd, ic, cfg);
                         state = ss.done ? kDone : kSearch;
                     }
                 }
