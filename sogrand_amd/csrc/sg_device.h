@@ -113,10 +113,9 @@ __device__ __forceinline__ double exp_neg_f32(float pm)
 
 // Per-lane LDS record columns of an NS-slot array: delta (Rec) and (H column |
 // natural position << 8) of sorted positions b = 0..15 at rows 1..16, and at
-// row 0 a null record (delta +0.0, H column 0) that the pattern loops read as
-// position -1, the index v_ffbl gives an exhausted mask, or as the empty field
-// (row 0) of a position word. The column pointers fold the slot in, so a
-// record address is one shift-add of b.
+// row 0 a null record (delta +0.0, H column 0) that the pattern rounds read
+// for the empty fields (row 0) of a position word. The column pointers fold the
+// slot in, so a record address is one shift-add of the row.
 template <class RecT, int NS> struct LineLdsT {
     using Rec = RecT;
     Rec *dcol;       // this slot's record of sorted position 0
@@ -231,13 +230,17 @@ struct ListEntries {
 // Set-up of a line whose magnitudes contain a tie, from d(i) = |x_i| in natural
 // order: the reference's quicksort decides the order among equal magnitudes
 // (replayed by quicksort_lds), then PM_HD in natural order and the sorted
-// records. Returns PM_HD.
+// records. Returns PM_HD; a_first and a_eighth receive the smallest and the
+// eighth smallest magnitude (the order statistics of IC, :619-621).
 template <class LDS>
-__device__ __forceinline__ double setup_with_ties(const LDS S, double jlim, const DecodeCfg &cfg)
+__device__ __forceinline__ double setup_with_ties(const LDS S, double jlim, const DecodeCfg &cfg,
+                                                  double &a_first, double &a_eighth)
 {
 #pragma unroll 1
     for (int i = 0; i < 16; ++i) S.h(i) = (uint16_t)(hcol_of((unsigned)i) | (i << 8));
     quicksort_lds(S);
+    a_first = S.d(0);
+    a_eighth = S.d(7);
     // natural-order PM_HD (:617): position i now sits at its sorted slot
     uint64_t slot_of = 0;
 #pragma unroll 1
@@ -321,28 +324,28 @@ __device__ __forceinline__ bool line_setup(const Src &src, const LDS &S, uint32_
     }
     if (!finite) return false;
 
+    // rank = the number of smaller magnitudes: a permutation exactly when the
+    // magnitudes are distinct, so a repeated rank (a missing bit in seen)
+    // reveals a tie, which the quicksort replay below orders
     uint64_t rank = 0; // 4 bits per natural position
-    bool tie = false;
+    uint32_t seen = 0;
     double a_first = 0.0, a_eighth = 0.0;
 #pragma unroll 1
     for (int i = 0; i < 16; ++i) {
         const double ai = S.d(i);
         uint32_t r = 0;
 #pragma unroll 4
-        for (int j = 0; j < 16; ++j) {
-            const double aj = S.d(j);
-            r += (aj < ai || (aj == ai && j < i)) ? 1u : 0u;
-            tie = tie || (aj == ai && j != i);
-        }
+        for (int j = 0; j < 16; ++j) r += S.d(j) < ai ? 1u : 0u;
         rank |= (uint64_t)r << (4 * i);
+        seen |= 1u << r;
         if (r == 0) a_first = ai;
         if (r == 7) a_eighth = ai;
     }
-    // the two order statistics of IC do not depend on how ties are arranged
-    ic = intercept(a_first, a_eighth, legacy);
+    const bool tie = seen != 0xffffu;
 
     pm_hd = 0.0;
     if (!tie) {
+        ic = intercept(a_first, a_eighth, legacy);
         // getPM_HD (:617) in natural order; delta_i (getPM's increment, :765)
 #pragma unroll 1
         for (int i = 0; i < 16; ++i) {
@@ -353,9 +356,10 @@ __device__ __forceinline__ bool line_setup(const Src &src, const LDS &S, uint32_
         }
         place_by_rank(S, rank);
     } else {
-        pm_hd = setup_with_ties(S, jlim, cfg);
+        pm_hd = setup_with_ties(S, jlim, cfg, a_first, a_eighth);
+        ic = intercept(a_first, a_eighth, legacy);
     }
-    S.d(-1) = 0.0; // null record read by exhausted masks in search_run
+    S.d(-1) = 0.0; // null record read by empty position-word fields
     S.h(-1) = 0;
     return true;
 }
@@ -405,7 +409,7 @@ __device__ __forceinline__ bool line_setup_f32(const Src &src, const LDS &S,
         S.d(i) = (float)(jaclog(ai, 50.0, cfg) - jn);
     }
     place_by_rank(S, rank);
-    S.d(-1) = 0.0f;
+    S.d(-1) = 0.0f; // null record
     S.h(-1) = 0;
     return true;
 }

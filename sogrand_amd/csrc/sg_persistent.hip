@@ -130,6 +130,17 @@ __device__ __forceinline__ uint32_t xor16(uint32_t v)
     return v;
 }
 
+// OR of v over each group of 16 lanes, in every lane of the group (the
+// pattern of xor16).
+__device__ __forceinline__ uint32_t or16(uint32_t v)
+{
+    v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, true);
+    v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, true);
+    v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x141, 0xF, 0xF, true);
+    v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x140, 0xF, 0xF, true);
+    return v;
+}
+
 // Four waves per SIMD asked for: the single-precision variant's records fit
 // four workgroups' LDS per CU and its registers fit 128 with a few bytes of
 // spill, which measures faster than three waves; the FP64 kernel's LDS allows
@@ -418,14 +429,21 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))
                     R.d(i) = key;
                 }
                 if (owner >= 0) {
-                    // rank by counting; in double, equal magnitudes (ties) go to
-                    // the owner, which replays the reference's quicksort; in the
-                    // single-precision variant equal keys are ordered by position
+                    // rank by counting. In double, the number of smaller
+                    // magnitudes, a permutation of 0..15 over the line's lanes
+                    // exactly when its magnitudes are distinct; a line with
+                    // equal magnitudes (a tie) goes to the owner, which replays
+                    // the reference's quicksort. In the single-precision variant
+                    // equal keys are ordered by position.
+                    if constexpr (P::kF32) {
 #pragma unroll 4
-                    for (int j = 0; j < 16; ++j) {
-                        const Rec aj = R.d(j);
-                        rank += (aj < key || (aj == key && j < i)) ? 1u : 0u;
-                        if (!P::kF32) tie = tie || (aj == key && j != i);
+                        for (int j = 0; j < 16; ++j) {
+                            const Rec aj = R.d(j);
+                            rank += (aj < key || (aj == key && j < i)) ? 1u : 0u;
+                        }
+                    } else {
+#pragma unroll 4
+                        for (int j = 0; j < 16; ++j) rank += R.d(j) < key ? 1u : 0u;
                     }
                     const double jn = jaclog(-ai, P::kJacLim, cfg);
                     delta = jaclog(ai, P::kJacLim, cfg) - jn; // getPM's increment (:765)
@@ -434,6 +452,8 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))
                     if (rank == 0) stage_a[2 * q] = (double)key;
                     if (rank == 7) stage_a[2 * q + 1] = (double)key;
                 }
+                // a repeated rank leaves one of the line's 16 rank bits unset
+                if constexpr (!P::kF32) tie = or16(owner >= 0 ? 1u << rank : 0u) != 0xffffu;
                 const uint64_t special = __ballot(owner >= 0 && (tie || !fin));
                 const uint64_t negm = __ballot(owner >= 0 && neg);
                 const uint64_t badm = __ballot(owner >= 0 && !fin);
@@ -454,7 +474,7 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))
                     pm_hd = 0.0;
 #pragma unroll
                     for (int j = 0; j < 16; ++j) pm_hd += stage_jn[16 * myq + j]; // getPM_HD (:617)
-                    // IC (:619-622)
+                    // IC (:619-622); a tie line's is taken from its sorted records
                     ic = intercept(stage_a[2 * myq], stage_a[2 * myq + 1], P::kLegacy);
                 }
                 // tasks of ordinary lines write the sorted records
@@ -463,7 +483,7 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))
                     R.h((int)rank) = (uint16_t)(hcol_of((unsigned)i) | (i << 8));
                 }
                 if (myq >= 0) {
-                    L.d(-1) = (Rec)0; // null record read by exhausted masks in search_run
+                    L.d(-1) = (Rec)0; // null record read by empty position-word fields
                     L.h(-1) = 0;
                     if (line_bad) {
                         atomicOr(&cnt[4 * my_slot + 3], 1u);
@@ -477,7 +497,11 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))
                     } else {
                         if constexpr (!P::kF32) {
                             // records hold |x| in natural order
-                            if (line_special) pm_hd = setup_with_ties(L, P::kJacLim, cfg);
+                            if (line_special) {
+                                double a_first, a_eighth;
+                                pm_hd = setup_with_ties(L, P::kJacLim, cfg, a_first, a_eighth);
+                                ic = intercept(a_first, a_eighth, P::kLegacy);
+                            }
                         }
                         search_init<P>(ss, ls, hd, s0, pm_hd, ic, cfg);
                         state = ss.done ? kDone : kSearch;

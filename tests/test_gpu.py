@@ -131,6 +131,28 @@ def test_cubes_2000_match_oracle(decoder, oracle, oracle_fma, backend):
         assert fg >= ff - 0.03
 
 
+@pytest.mark.parametrize("backend", [0, 1], ids=["plain", "persistent"])
+def test_cubes_with_ties_match_oracle(decoder, oracle, oracle_fma, backend):
+    # channel LLRs on a grid of 1/4: most lines start with equal magnitudes, so
+    # the set-up's tie detection and the quicksort replay run in every kernel
+    sg = _sg()
+    llr = np.round(sg.gen_llr(2024, 64, ebn0_db=2.5) * 4.0) / 4.0
+    lines = np.abs(llr.reshape(-1, 16))
+    ties = np.array([len(np.unique(v)) < 16 for v in lines])
+    assert ties.mean() > 0.9
+    decoder.set_params(backend=backend)
+    r = decoder.decode(llr, soft=True)
+    bits, st, soft, cond = oracle.decode(llr, soft=True, cond=True)
+    bits_f, st_f, _ = oracle_fma.decode(llr)
+    stable = stable_codewords(st, st_f, bits, bits_f)
+    _say("ties: oracle builds agree on", int(stable.sum()), "of", len(llr))
+    assert stable.mean() >= 0.9
+    assert (r.bits[stable] == bits[stable]).all()
+    assert (r.stats[stable] == st[stable]).all()
+    well = cond <= 1e-11
+    assert soft_close(r.soft[well], soft[well])
+
+
 def test_backends_identical_and_batch_independent(decoder):
     sg = _sg()
     rng = np.random.default_rng(5)
