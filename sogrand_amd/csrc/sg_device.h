@@ -132,16 +132,6 @@ template <class RecT, int NS> struct LineLdsT {
     __device__ uint16_t hr(uint32_t row) const { return hcol[((int)row - 1) * NS]; }
 };
 
-// Index of the lowest set bit, -1 for 0 (v_ffbl_b32). Written as the
-// instruction itself: as a C expression the compiler re-creates the zero case
-// with a select once the index feeds an address computation.
-__device__ __forceinline__ int lowest_bit(uint32_t x)
-{
-    int b;
-    asm("v_ffbl_b32 %0, %1" : "=v"(b) : "v"(x));
-    return b;
-}
-
 // JacLog (:762; cut at lim = 50, or 30 in cubic_decoder.c:699).
 __device__ __forceinline__ double jaclog(double x, double lim, const DecodeCfg &cfg)
 {
