@@ -73,8 +73,8 @@ typedef struct sogrand_params {
     int32_t waves_per_cu; /* persistent: resident waves per CU (0 = as many as fit)      */
     int32_t lane_batch; /* persistent: free lanes gathered before new lines are handed out
                            (0 = tuned default)                                            */
-    int32_t step_len;   /* persistent: guesses per lane between scheduling points
-                           (0 = tuned default)                                            */
+    int32_t step_len;   /* persistent: guesses per lane between scheduling points, taken
+                           in steps of 16 (0 = tuned default)                             */
     int32_t use_graph;  /* 1: when every buffer of a decode is device memory, capture its
                            launch sequence in a hipGraph once per (buffers, size,
                            parameters) and replay it on later calls                      */

@@ -677,12 +677,12 @@ __device__ __forceinline__ void search_pair(SearchState &ss, LineList &ls, const
                              [&](int j) { return (uint32_t)masks[t + (uint32_t)j]; });
 }
 
-// Further guesses of the schedule, one chunk of 8 position words (four pairs,
-// 32 bytes) per iteration, up to about max_steps guesses. A search starts at a
-// chunk boundary and a lane that stops mid-chunk is done, so every searching
-// lane enters an iteration at a chunk boundary; the chunk was loaded one
-// iteration ahead. The loop runs an iteration count common to the wave and
-// leaves when no lane is still searching; a finished lane idles inside it,
+// Further guesses of the schedule, in chunks of 8 position words (four pairs,
+// 32 bytes), up to about max_steps guesses (rounded up to 16). A search starts
+// at a chunk boundary and a lane that stops mid-chunk is done, so every
+// searching lane enters each half-iteration at a chunk boundary; each chunk is
+// loaded one chunk ahead. The loop runs an iteration count common to the wave
+// and leaves when no lane is still searching; a finished lane idles inside it,
 // which keeps the wave's control flow to one mask.
 template <bool FINITE_T, class P, class LDS>
 __device__ __forceinline__ void search_run(SearchState &ss, LineList &ls, const LDS &S, uint32_t hd,
@@ -690,31 +690,37 @@ __device__ __forceinline__ void search_run(SearchState &ss, LineList &ls, const 
                                            const DecodeCfg &cfg, uint32_t max_steps)
 {
 #pragma unroll 1
-    for (uint32_t step = 0; step < max_steps; step += 8) {
+    for (uint32_t step = 0; step < max_steps; step += 16) {
+        // two chunks per iteration, in alternating registers: the chunk for
+        // the second half is loaded at the start of the first, the next
+        // iteration's first chunk at the start of the second, each into
+        // registers whose previous value is already consumed, so the load
+        // lands in place and is waited for only a chunk later (with one
+        // chunk per iteration the compiler copied the prefetched registers
+        // after the load, which waited for it at once)
         if (__ballot(!ss.done) == 0ull) break;
-        if (ss.done) continue;
-        const uint32_t t = ss.t; // a multiple of 8 here
-        // The prefetched chunk is copied out by explicit moves ordered before
-        // the next load. Left to the compiler, the copy becomes a phi copy at
-        // the end of the block, after the load, so the load lands in a
-        // temporary and is moved into the prefetch registers at once, which
-        // waits for it and defeats the prefetch.
-        uint4 c, d;
-        asm volatile("v_mov_b32 %0, %1" : "=v"(c.x) : "v"(ss.nxt.x) : "memory");
-        asm volatile("v_mov_b32 %0, %1" : "=v"(c.y) : "v"(ss.nxt.y) : "memory");
-        asm volatile("v_mov_b32 %0, %1" : "=v"(c.z) : "v"(ss.nxt.z) : "memory");
-        asm volatile("v_mov_b32 %0, %1" : "=v"(c.w) : "v"(ss.nxt.w) : "memory");
-        asm volatile("v_mov_b32 %0, %1" : "=v"(d.x) : "v"(ss.nxt2.x) : "memory");
-        asm volatile("v_mov_b32 %0, %1" : "=v"(d.y) : "v"(ss.nxt2.y) : "memory");
-        asm volatile("v_mov_b32 %0, %1" : "=v"(d.z) : "v"(ss.nxt2.z) : "memory");
-        asm volatile("v_mov_b32 %0, %1" : "=v"(d.w) : "v"(ss.nxt2.w) : "memory");
-        const uint4 *np = ss.tp + (t >> 2) + 2; // tables carry one chunk of padding
-        ss.nxt = np[0];
-        ss.nxt2 = np[1];
-        search_pair<FINITE_T, P>(ss, ls, S, s0, pm_hd, ic, cfg, c.x, c.y, t);
-        if (!ss.done) search_pair<FINITE_T, P>(ss, ls, S, s0, pm_hd, ic, cfg, c.z, c.w, t + 2u);
-        if (!ss.done) search_pair<FINITE_T, P>(ss, ls, S, s0, pm_hd, ic, cfg, d.x, d.y, t + 4u);
-        if (!ss.done) search_pair<FINITE_T, P>(ss, ls, S, s0, pm_hd, ic, cfg, d.z, d.w, t + 6u);
+        uint4 b0, b1;
+        if (!ss.done) {
+            const uint32_t t = ss.t; // a multiple of 8 here
+            const uint4 *np = ss.tp + (t >> 2) + 2; // tables carry one chunk of padding
+            b0 = np[0];
+            b1 = np[1];
+            search_pair<FINITE_T, P>(ss, ls, S, s0, pm_hd, ic, cfg, ss.nxt.x, ss.nxt.y, t);
+            if (!ss.done) search_pair<FINITE_T, P>(ss, ls, S, s0, pm_hd, ic, cfg, ss.nxt.z, ss.nxt.w, t + 2u);
+            if (!ss.done) search_pair<FINITE_T, P>(ss, ls, S, s0, pm_hd, ic, cfg, ss.nxt2.x, ss.nxt2.y, t + 4u);
+            if (!ss.done) search_pair<FINITE_T, P>(ss, ls, S, s0, pm_hd, ic, cfg, ss.nxt2.z, ss.nxt2.w, t + 6u);
+        }
+        if (__ballot(!ss.done) == 0ull) break;
+        if (!ss.done) {
+            const uint32_t t = ss.t;
+            const uint4 *np = ss.tp + (t >> 2) + 2;
+            ss.nxt = np[0];
+            ss.nxt2 = np[1];
+            search_pair<FINITE_T, P>(ss, ls, S, s0, pm_hd, ic, cfg, b0.x, b0.y, t);
+            if (!ss.done) search_pair<FINITE_T, P>(ss, ls, S, s0, pm_hd, ic, cfg, b0.z, b0.w, t + 2u);
+            if (!ss.done) search_pair<FINITE_T, P>(ss, ls, S, s0, pm_hd, ic, cfg, b1.x, b1.y, t + 4u);
+            if (!ss.done) search_pair<FINITE_T, P>(ss, ls, S, s0, pm_hd, ic, cfg, b1.z, b1.w, t + 6u);
+        }
     }
 }
 
