@@ -9,8 +9,10 @@
 // Numerics follow SOGRAND_C/cubic_decoder_final.c exactly (citations are to
 // that file unless another is named): every double is formed by the same
 // operations in the same order; the file is compiled with -ffp-contract=off so
-// nothing is fused. The policy PrecF64Legacy follows the reference's earlier
-// SOGRAND_C/cubic_decoder.c instead.
+// nothing is fused. Where the reference calls the C library's exp and log,
+// the decoder uses its own table-driven ones (sg_exp.h, sg_log.h: within 0.55
+// and 0.502 ulp, as faithful as the library's). The policy PrecF64Legacy
+// follows the reference's earlier SOGRAND_C/cubic_decoder.c instead.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -24,11 +26,11 @@
 
 namespace sg {
 
-
 struct DecodeCfg {
     uint64_t tmax;          // 0 already mapped to 0x7fffffff by the host
     double thres;           // 1 - p_ET
-    const uint16_t *sched;  // kSchedTables x kSchedStride masks
+    const uint16_t *sched;  // schedule buffer: kSchedTables x kSchedStride masks, then
+                            // the position words (pos_tables, sg_common.h)
     uint32_t L;             // list size 1..kMaxL
     const ExpEntry *exptab; // exp table in LDS (load_exp_table)
     const LogEntry *logtab; // log table in LDS (load_log_table)
@@ -683,7 +685,7 @@ __device__ __forceinline__ void search_pair(SearchState &ss, LineList &ls, const
 // searching lane enters each half-iteration at a chunk boundary; each chunk is
 // loaded one chunk ahead. The loop runs an iteration count common to the wave
 // and leaves when no lane is still searching; a finished lane idles inside it,
-// which keeps the wave's control flow to one mask.
+// which keeps the wave's control flow to one exec mask.
 template <bool FINITE_T, class P, class LDS>
 __device__ __forceinline__ void search_run(SearchState &ss, LineList &ls, const LDS &S, uint32_t hd,
                                            uint32_t s0, double pm_hd, int32_t ic,
@@ -736,13 +738,13 @@ __device__ __forceinline__ void line_search(const LDS &S, uint32_t hd,
 }
 
 // Soft output of position i (SOGRAND_bitSO :536-575) given the line input x.
-// eps: EPSILON of the clamps (1e-40, :16; DBL_EPSILON in cubic_decoder_updated.c:16;
-// 1e-9 in cubic_decoder.c:509-510); eps_log: floor inside the logs (EPSILON, or
-// 1e-30 in cubic_decoder.c:533).
+// cfg.eps: EPSILON of the clamps (1e-40, :16; DBL_EPSILON in
+// cubic_decoder_updated.c:16; 1e-9 in cubic_decoder.c:509-510); cfg.eps_log:
+// floor inside the logs (EPSILON, or 1e-30 in cubic_decoder.c:533).
 __device__ __forceinline__ double soft_app(const ListEntries &ls, int nl, double pnotin, int i,
-                                           double x, double eps, double eps_log,
-                                           const DecodeCfg &cfg)
+                                           double x, const DecodeCfg &cfg)
 {
+    const double eps = cfg.eps, eps_log = cfg.eps_log;
     double pp1 = 1.0 / (1.0 + exp_any(x, cfg.exptab));
     double pp0 = 1.0 - pp1;
     pp1 = fmin(fmax(pp1, eps), 1.0 - eps);

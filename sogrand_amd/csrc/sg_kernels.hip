@@ -131,7 +131,7 @@ __global__ __launch_bounds__(kPlainThreads) void plain_kernel(CubeJob job)
                         la = xin;
                         lenew = 0.0;
                     } else {
-                        la = soft_app(ent, ls.nl, pnotin, i, xin, cfg.eps, cfg.eps_log, cfg);
+                        la = soft_app(ent, ls.nl, pnotin, i, xin, cfg);
                         lenew = la - xin;
                     }
                     le[p] = lenew;
@@ -237,7 +237,7 @@ __global__ __launch_bounds__(256) void lines_kernel(const double *llr, size_t n,
             a = xi;
             e = 0.0;
         } else {
-            a = soft_app(ent, ls.nl, pnotin, i, xi, cfg.eps, cfg.eps_log, cfg);
+            a = soft_app(ent, ls.nl, pnotin, i, xi, cfg);
             e = a - xi;
         }
         if (la) la[line * 16 + i] = a;

@@ -364,8 +364,7 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))
                         la = xin;
                         lenew = 0.0;
                     } else {
-                        la = soft_app(ent, ol.nl, fmax(ol.pnl, cfg.eps), i, xin, cfg.eps,
-                                      cfg.eps_log, cfg);
+                        la = soft_app(ent, ol.nl, fmax(ol.pnl, cfg.eps), i, xin, cfg);
                         lenew = la - xin;
                     }
                     le[p] = lenew;
@@ -448,7 +447,8 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))
                     const double jn = jaclog(-ai, P::kJacLim, cfg);
                     delta = jaclog(ai, P::kJacLim, cfg) - jn; // getPM's increment (:765)
                     stage_jn[lane] = jn;
-                    // the two order statistics of IC do not depend on tie order
+                    // the two order statistics of IC (a tie line's IC is taken
+                    // from its sorted records instead)
                     if (rank == 0) stage_a[2 * q] = (double)key;
                     if (rank == 7) stage_a[2 * q + 1] = (double)key;
                 }
