@@ -10,7 +10,7 @@
 // that file unless another is named): every double is formed by the same
 // operations in the same order; the file is compiled with -ffp-contract=off so
 // nothing is fused. Where the reference calls the C library's exp and log,
-// the decoder uses its own table-driven ones (sg_exp.h, sg_log.h: within 0.55
+// the decoder uses its own table-driven ones (sg_exp.h, sg_log.h: within 0.54
 // and 0.502 ulp, as faithful as the library's). The policy PrecF64Legacy
 // follows the reference's earlier SOGRAND_C/cubic_decoder.c instead.
 #pragma once
@@ -39,7 +39,7 @@ struct DecodeCfg {
                             // cubic_decoder.c:533)
 };
 
-// Fill the LDS exp table (first 34 threads) - the caller synchronises.
+// Fill the LDS exp table (first kExpTableEntries threads) - the caller synchronises.
 __device__ __forceinline__ void load_exp_table(ExpEntry *lds, int tid)
 {
     if (tid < kExpTableEntries) lds[tid] = exp_table_entry(tid);
@@ -54,7 +54,7 @@ __device__ __forceinline__ void load_log_table(LogEntry *lds, int tid)
 
 // exp(-pm) for pm >= 0, the probability weight of one guess (:626, :672, :712).
 // pm is capped at 746 (which also maps +inf there) to keep k in int range: at
-// the cap k >> 5 = -1077 and m < 2, so the result is 0.
+// the cap k >> 6 = -1077 and m < 2, so the result is 0.
 __device__ __forceinline__ double exp_neg(double pm, const ExpEntry *tab)
 {
     // the cap as the instruction itself: as fmin the compiler first quiets a
@@ -66,7 +66,7 @@ __device__ __forceinline__ double exp_neg(double pm, const ExpEntry *tab)
 }
 
 // exp(x) for any finite x: capped to [-746, 710], where the result is 0 or
-// overflows to +inf as exp's does (at 710, k >> 5 = 1024 and m > 1).
+// overflows to +inf as exp's does (at 710, k >> 6 = 1024 and m > 1).
 __device__ __forceinline__ double exp_any(double x, const ExpEntry *tab)
 {
     double xc;
