@@ -5,6 +5,7 @@
 // device evaluate every double expression with identical rounding.
 #pragma once
 
+#include <stddef.h>
 #include <stdint.h>
 
 #if defined(__HIPCC__)

@@ -55,3 +55,16 @@ def test_exp_accuracy(accuracy):
     assert max_e4 <= 5500
     assert vs_expl <= 0.01 * n
     assert vs_libc <= 0.01 * n
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="g++ not found")
+def test_position_words(tmp_path):
+    # the schedule's position words against their masks, and the claim that one
+    # generic table's weight levels serve every intercept from 56 on
+    exe = str(tmp_path / "pos_check")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-I", CSRC,
+                    os.path.join(ROOT, "tests", "pos_check.cpp"),
+                    os.path.join(CSRC, "sg_schedule.cpp"), "-o", exe],
+                   check=True, capture_output=True)
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=600)
+    assert out.returncode == 0 and out.stdout.strip() == "ok", out.stdout
