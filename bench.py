@@ -44,6 +44,7 @@ if int(os.environ.get("GPU_MAX_HW_QUEUES", "0") or 0) < 16:
 
 METRIC = "codewords/s @Eb/N0 2 dB (BER-matched), batched vs plain; p99 latency"
 VALU_PEAK = 256 * 4 * 16 * 2.4e9  # lane-ops/s: CUs x SIMDs x lanes x clock
+HBM_PEAK = 8.0e12  # bytes/s, MI355X HBM3E
 C_GUESS = 36.0   # VALU lane-ops per guess (algorithmic budget)
 C_LINE = 3500.0  # VALU lane-ops per component decode (set-up + soft output)
 PUBLISHED_CW_S = 13.19  # the reference's published GPU throughput at 2 dB (README.md:137-143)
@@ -563,6 +564,7 @@ def workload_headline(args) -> dict:
     lines = 256.0 * float(st["half_iters"].astype(np.float64).sum())
     kernel_s = (sum(kmss) / len(kmss)) / 1e3
     achieved = (C_GUESS * ng + C_LINE * lines) / kernel_s
+    traffic = measured_traffic(n, args.ebn0)
     out = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -595,9 +597,18 @@ def workload_headline(args) -> dict:
             "peak": round(VALU_PEAK / 1e12, 2),
             "unit": "Tops/s (VALU lane-ops)",
             "frac": round(achieved / VALU_PEAK, 4),
-            "traffic": measured_traffic(n, args.ebn0),
+            "traffic": traffic,
             "per_unit": {"lane_ops_per_guess": C_GUESS, "lane_ops_per_line_decode": C_LINE},
             "kernel_ms": round(kernel_s * 1e3, 2),
+            # the same launch against HBM, to show that bandwidth does not bound
+            # it: the measured bytes (traffic) and the algorithmic stream (32 KB
+            # of LLRs in, 64 B of bits out per codeword) over the launch time
+            "hbm": {
+                "measured_GB_s": round(traffic / kernel_s / 1e9, 1) if traffic else None,
+                "algorithmic_GB_s": round(n * (4096 * 8 + 64) / kernel_s / 1e9, 1),
+                "peak_GB_s": HBM_PEAK / 1e9,
+                "frac": round(traffic / kernel_s / HBM_PEAK, 4) if traffic else None,
+            },
         },
     }
     if not args.no_serve:
