@@ -65,43 +65,54 @@ def cpu_model() -> str:
     return platform.processor()
 
 
+# The restatement's speed relative to the reference itself, measured in the
+# build container on one core (same 40 codewords at 2 dB, identical bits):
+# SOGRAND_C cubic_decoder_final.c built -O2 decodes REF_PER_CORE_CW_S codewords/s,
+# the restatement ORACLE_PER_CORE_CW_S.
+REF_PER_CORE_CW_S = 1.60
+ORACLE_PER_CORE_CW_S = 22.81
+
+
+def host_cores() -> int:
+    """The host cores this process may run on (its CPU affinity)."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
 def cpu_baseline(seed: int, ebn0: float, seconds: float) -> dict:
-    """SOGRAND_C timed on the host cores over a bounded sample of the same workload."""
+    """The C restatement of SOGRAND_C (oracle/liboracle.so, pinned bit for bit to
+    the reference's own outputs) timed on every host core of this process, one
+    codeword per thread, over a bounded sample of the same workload."""
     import sogrand_amd as sg
 
-    cores = min(16, os.cpu_count() or 1)
-    ref = os.path.join(ROOT, "oracle", "_ref", "cubic_decoder_final_O2")
-    if os.path.exists(ref):
-        per = max(1, int(round(seconds * 4.5)))  # ~4.5 cw/s per EPYC core at 2 dB (-O2)
-        n = per * cores
-        llr = sg.gen_llr(seed, n, ebn0_db=ebn0)
-        with tempfile.TemporaryDirectory() as d:
-            files = []
-            for p in range(cores):
-                fi = os.path.join(d, f"in{p}.bin")
-                llr[p * per:(p + 1) * per].astype("<f8").tofile(fi)
-                files.append(fi)
-            t0 = time.perf_counter()
-            procs = [subprocess.Popen([ref, fi, fi + ".out"], stdout=subprocess.DEVNULL)
-                     for fi in files]
-            rcs = [p.wait() for p in procs]
-            dt = time.perf_counter() - t0
-        if all(r == 0 for r in rcs):
-            return {"value": n / dt, "unit": "codewords/s", "cores": cores, "kind": "reference",
-                    "sample": f"{n} codewords of the same workload ({per} per process, "
-                              f"{cores} processes of SOGRAND_C cubic_decoder_final built -O2), "
-                              f"{dt:.1f} s on {cpu_model()}"}
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from oracle import Oracle
 
-    n = max(cores, int(seconds * 15 * cores))
+    cores = host_cores()
+    o = Oracle()
+    # size the sample from a short single-codeword probe on all cores
+    probe = sg.gen_llr(seed + 7, cores, ebn0_db=ebn0)
+    t0 = time.perf_counter()
+    o.decode(probe, threads=cores)
+    rate = cores / max(time.perf_counter() - t0, 1e-3)
+    n = max(cores, int(seconds * rate))
     llr = sg.gen_llr(seed, n, ebn0_db=ebn0)
     t0 = time.perf_counter()
-    Oracle().decode(llr, threads=cores)
+    o.decode(llr, threads=cores)
     dt = time.perf_counter() - t0
-    return {"value": n / dt, "unit": "codewords/s", "cores": cores, "kind": "port",
-            "sample": f"{n} codewords of the same workload, C restatement of SOGRAND_C, "
-                      f"{dt:.1f} s on {cpu_model()}"}
+    out = {"value": round(n / dt, 2), "unit": "codewords/s", "cores": cores, "kind": "port",
+           "sample": f"{n} codewords of the same workload (Eb/N0 {ebn0} dB), C restatement of "
+                     f"SOGRAND_C built -O2 without FMA, {cores} threads, {dt:.1f} s on "
+                     f"{cpu_model()}"}
+    if REF_PER_CORE_CW_S and ORACLE_PER_CORE_CW_S:
+        out["calibration"] = (f"per core, the restatement decodes {ORACLE_PER_CORE_CW_S} cw/s "
+                              f"where SOGRAND_C itself (-O2) decodes {REF_PER_CORE_CW_S} "
+                              f"(build container, 2 dB): the reference would reach about "
+                              f"{round(n / dt * REF_PER_CORE_CW_S / ORACLE_PER_CORE_CW_S, 1)} "
+                              f"cw/s on these cores")
+    return out
 
 
 class _View:
@@ -174,6 +185,7 @@ def serve_load(dec, capacity: float, load: float, seconds: float, seed: int,
     from a frame's scheduled arrival to its results being back in host memory."""
     import sogrand_amd as sg
 
+    require_persistent(dec)
     rng = np.random.default_rng(seed)
     mean_frame = 128.5
     lam = load * capacity / mean_frame  # frames/s
@@ -230,10 +242,21 @@ def serve_load(dec, capacity: float, load: float, seconds: float, seed: int,
             "max_client_lag_ms": round(1e3 * lag, 3), "bler": blk / n}
 
 
+def require_persistent(dec) -> None:
+    """The serving leg measures the batched (persistent) kernel: refuse a decoder
+    left on another backend (the server copies the context's parameters)."""
+    import sogrand_amd as sg
+
+    if dec.params.backend != sg.BACKEND_PERSISTENT:
+        raise ValueError(f"serving runs on the persistent backend, not {dec.params.backend}")
+
+
 def serving_capacity(dec, seed: int, n: int = 40000) -> float:
     """Batched codewords/s on the serving frame mix (per-codeword Eb/N0 ~ U[1, 3] dB)."""
     import sogrand_amd as sg
     from sogrand_amd import hip
+
+    require_persistent(dec)
 
     snr = np.random.default_rng(seed).uniform(1.0, 3.0, n)
     d_llr = gen_device(dec, seed, n, [sg.sigma_for_ebn0(float(e)) for e in snr])
@@ -499,18 +522,36 @@ def workload_copy(args) -> dict:
             "bytes_written_per_copy": nbytes, "copies": 3}
 
 
+def kernel_source_hash() -> str:
+    """SHA-256 over the decoder's device and host sources and build flags: the
+    identity of the build a counter run measured."""
+    import hashlib
+
+    from sogrand_amd import build as b
+
+    h = hashlib.sha256(" ".join(b.COMMON).encode())
+    for name in sorted(os.listdir(b.CSRC)):
+        if name.endswith((".hip", ".h", ".cpp")):
+            with open(os.path.join(b.CSRC, name), "rb") as f:
+                h.update(name.encode() + b"\0" + f.read())
+    return h.hexdigest()
+
+
 def measured_traffic(n: int, ebn0: float):
     """HBM bytes per launch of the persistent kernel on this workload, from the
     committed counter runs (FETCH_SIZE and WRITE_SIZE scaled by a copy kernel
-    of known size); None for another workload."""
+    of known size) -> (bytes or None, note). None for another workload, or when
+    the record was measured on other sources than the ones built here."""
     path = os.path.join(ROOT, "profiles", "persistent_100k_cw_2dB_traffic.json")
     try:
         t = json.load(open(path))
     except (OSError, ValueError):
-        return None
+        return None, "no counter record"
     if t.get("n_cw") != n or t.get("ebn0_db") != ebn0:
-        return None
-    return round(t["traffic_bytes_per_launch"])
+        return None, "no counter record for this workload"
+    if t.get("source_sha256") != kernel_source_hash():
+        return None, "counter record is stale (measured on other decoder sources)"
+    return round(t["traffic_bytes_per_launch"]), f"profiles/{os.path.basename(path)}"
 
 
 # ---------------------------------------------------------------------------
@@ -530,7 +571,10 @@ def workload_headline(args) -> dict:
 
     def run(backend: int) -> tuple[float, float]:
         dec.set_params(backend=backend)
-        return timed_decode(dec, d_llr, d_bits, d_stats)
+        try:
+            return timed_decode(dec, d_llr, d_bits, d_stats)
+        finally:
+            dec.set_params(backend=sg.BACKEND_PERSISTENT)
 
     for w in range(args.warmup):
         run(sg.BACKEND_PERSISTENT)
@@ -564,7 +608,7 @@ def workload_headline(args) -> dict:
     lines = 256.0 * float(st["half_iters"].astype(np.float64).sum())
     kernel_s = (sum(kmss) / len(kmss)) / 1e3
     achieved = (C_GUESS * ng + C_LINE * lines) / kernel_s
-    traffic = measured_traffic(n, args.ebn0)
+    traffic, traffic_note = measured_traffic(n, args.ebn0)
     out = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -598,6 +642,7 @@ def workload_headline(args) -> dict:
             "unit": "Tops/s (VALU lane-ops)",
             "frac": round(achieved / VALU_PEAK, 4),
             "traffic": traffic,
+            "traffic_source": traffic_note,
             "per_unit": {"lane_ops_per_guess": C_GUESS, "lane_ops_per_line_decode": C_LINE},
             "kernel_ms": round(kernel_s * 1e3, 2),
             # the same launch against HBM, to show that bandwidth does not bound
@@ -620,7 +665,9 @@ def workload_headline(args) -> dict:
                           "workload": "serving loop, Poisson frames of U{1..256} codewords at "
                                       "Eb/N0 ~ U[1, 3] dB, offered 70 % of batched capacity",
                           "capacity_cw_s": round(cap, 1), "sustained_cw_s": r["sustained_cw_s"],
-                          "frames": r["frames"]}
+                          "frames": r["frames"],
+                          "backend": "persistent" if dec.params.backend == sg.BACKEND_PERSISTENT
+                          else str(dec.params.backend)}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.seed, args.ebn0, args.cpu_seconds)
     return out

@@ -1,75 +1,69 @@
-"""Generate sg_exp_table.h: 2^(j/64) as exact hi + lo double pairs, the ln2/64
-split used by the device exp, and the coefficients of its polynomial.
+"""Generate sg_exp_table.h: the table and constants of the decoder's exp.
 
-The polynomial: exp(r) - 1 = r + r^2 (c2 + c3 r + c4 r^2 + c5 r^3) for
-|r| <= ln2/128, the cubic interpolating (exp(r) - 1 - r)/r^2 at the four
-Chebyshev nodes of that interval (near-minimax; truncation below 2^-57.6
-relative to exp(r)).
+The method is the table-driven exp of Arm Optimized Routines (Szabolcs Nagy,
+2018; MIT / Apache-2.0), which is also the exp of the C library the reference
+calls (glibc 2.28 and later, sysdeps/ieee754/dbl-64/e_exp.c). With N = 128:
+
+    x = k ln2/N + r, |r| <= ln2/(2N),   k = round(x N/ln2)
+    2^(k/N) = 2^(k>>7) * H[j] * (1 + T[j]),   j = k mod 128
+    exp(x) = scale + scale * (T[j] + r + r^2 (C2 + r C3) + r^4 (C4 + r C5))
+
+The table holds, per j, the tail T[j] = ((2^(j/N) - H[j]) / H[j]) rounded, with
+H[j] = 2^(j/N) rounded to double, and the bits of H[j] minus j << 45, so that
+adding k << 45 to them gives the bits of 2^(k/N) rounded. Both are computed
+here with 80-digit decimal arithmetic. The reduction constants and the
+polynomial coefficients C2..C5 are the published ones of that design.
 
     python sogrand_amd/csrc/make_exp_table.py > sogrand_amd/csrc/sg_exp_table.h
 """
-import math
+import struct
 from decimal import Decimal, getcontext
 
-getcontext().prec = 60
-N = 64
+getcontext().prec = 80
+N = 128
 
 
-def cubic_coefficients():
-    h = Decimal(2).ln() / (2 * N)
-    nodes = [h * Decimal(math.cos((2 * k + 1) * math.pi / 8)) for k in range(4)]
-
-    def power(x, j):
-        out = Decimal(1)
-        for _ in range(j):
-            out *= x
-        return out
-
-    m = [[power(x, j) for j in range(4)] + [(x.exp() - 1 - x) / (x * x)] for x in nodes]
-    for c in range(4):  # Gauss-Jordan elimination with partial pivoting
-        piv = max(range(c, 4), key=lambda r: abs(m[r][c]))
-        m[c], m[piv] = m[piv], m[c]
-        for r in range(4):
-            if r != c:
-                f = m[r][c] / m[c][c]
-                m[r] = [m[r][k] - f * m[c][k] for k in range(5)]
-    return [float(m[i][4] / m[i][i]) for i in range(4)]
+def bits(x: float) -> int:
+    return struct.unpack("<Q", struct.pack("<d", x))[0]
 
 
 def main() -> None:
     rows = []
     for j in range(N):
         v = Decimal(2) ** (Decimal(j) / N)
-        hi = float(v)
-        lo = float(v - Decimal(hi))
-        rows.append(f"    {{{hi.hex()}, {lo.hex()}}}, // 2^({j}/{N})")
-    l1 = float(Decimal(2).ln() / N)
-    l2 = float(Decimal(2).ln() / N - Decimal(l1))
-    inv = float(N / Decimal(2).ln())
-    c2, c3, c4, c5 = cubic_coefficients()
+        h = float(v)
+        tail = float((v - Decimal(h)) / Decimal(h))
+        sbits = (bits(h) - (j << 45)) & 0xFFFFFFFFFFFFFFFF
+        rows.append(f"    {{{tail.hex()}, 0x{sbits:016x}ull}}, // 2^({j}/{N})")
     body = "\n".join(rows)
-    print(f"""// Table for the device exp: 2^(j/{N}) = hi + lo to about 106 bits, the
-// reduction constants and the polynomial coefficients, generated with 60-digit
-// decimal arithmetic by make_exp_table.py.
+    print(f"""// Table of the decoder's exp (the table-driven exp of Arm Optimized Routines,
+// N = {N}): per j, the tail T[j] of 2^(j/{N}) = H[j] (1 + T[j]) and the bits of
+// H[j] minus j << 45, generated with 80-digit decimal arithmetic by
+// make_exp_table.py; the reduction constants and polynomial coefficients of
+// that design.
 #pragma once
+
+#include <stdint.h>
 
 namespace sg {{
 
-constexpr int kExpTableBits = 6;
-constexpr double kExpInvLn2N = {inv.hex()}; // {N} / ln 2
-constexpr double kExpLn2NHi = {l1.hex()};   // ln 2 / {N}, rounded
-constexpr double kExpLn2NLo = {l2.hex()};   // ln 2 / {N} - kExpLn2NHi
-// exp(r) - 1 = r + r^2 (c2 + c3 r + c4 r^2 + c5 r^3), |r| <= ln 2 / {2 * N}
-constexpr double kExpC2 = {c2.hex()};
-constexpr double kExpC3 = {c3.hex()};
-constexpr double kExpC4 = {c4.hex()};
-constexpr double kExpC5 = {c5.hex()};
+constexpr int kExpTableBits = 7;
+constexpr double kExpInvLn2N = 0x1.71547652b82fep0 * {N}; // {N} / ln 2
+constexpr double kExpNegLn2HiN = -0x1.62e42fefa0000p-8;  // -ln 2 / {N}, high part
+constexpr double kExpNegLn2LoN = -0x1.cf79abc9e3b3ap-47; // -ln 2 / {N}, low part
+constexpr double kExpShift = 0x1.8p52;                   // round-to-integer shifter
+// exp(r) - 1 - T ~ r + r^2 (C2 + r C3) + r^4 (C4 + r C5), |r| <= ln 2 / {2 * N}
+constexpr double kExpC2 = 0x1.ffffffffffdbdp-2;
+constexpr double kExpC3 = 0x1.555555555543cp-3;
+constexpr double kExpC4 = 0x1.55555cf172b91p-5;
+constexpr double kExpC5 = 0x1.1111167a4d017p-7;
 
-struct ExpPair {{
-    double hi, lo;
+struct ExpTail {{
+    double tail;
+    uint64_t sbits;
 }};
 
-constexpr ExpPair kExp2Table[{N}] = {{
+constexpr ExpTail kExp2Table[{N}] = {{
 {body}
 }};
 

@@ -1,53 +1,216 @@
-"""Generate sg_log_table.h: the table and constants of the device logarithm
-(table-driven, after P. T. P. Tang, "Table-driven implementation of the
-logarithm function in IEEE floating-point arithmetic", ACM TOMS 16(4), 1990).
+"""Generate sg_log_table.h: the table and constants of the decoder's log.
 
-For c = 1 + i/128 (i = -37 .. 53, covering mantissas in [sqrt(1/2), sqrt(2))):
-invc = 1/c rounded to double, and -log(invc) split as logc_hi (a multiple of
-2^-42, so that k ln2_hi + logc_hi is exact for |k| <= 1075) plus logc_lo.
+The method is the table-driven log of Arm Optimized Routines (Szabolcs Nagy,
+2018; MIT / Apache-2.0), which is also the log of the C library the reference
+calls (glibc 2.28 and later, sysdeps/ieee754/dbl-64/e_log.c). x = 2^k z with z
+in [0x1.6p-1, 0x1.6p0) split into 128 subintervals by the top bits of z; for
+subinterval i, log(x) = k ln2 + logc_i + log1p(z invc_i - 1), where
+logc_i = log(1/invc_i) rounded and log1p(r) - r is a polynomial. Near x = 1
+(0x1.ep-1 <= x < 0x1.1090000000000p0) a separate polynomial in x - 1 is used.
+The polynomial coefficients and ln 2 split are the published ones of that
+design.
+
+The table (invc_i, logc_i) is rebuilt here by the design's documented
+conditions: invc_i is a double within 2^29 ulps of 1/center_i such that
+log(1/invc_i) lies within 2^-66 of a multiple of 2^-43 (so that logc_i is that
+multiple, k ln2_hi + logc_i is exact, and logc_i errs by < 2^-66). About 250
+doubles per subinterval meet them (found by a forward-difference scan of
+log(1/invc) in 128-bit fixed point and confirmed with 80-digit decimal
+arithmetic); the design's further selection among them is reproduced by
+keeping the one candidate whose log agrees bit for bit with this platform's C
+library on 50,000 arguments of the subinterval, and that candidate is then
+confirmed on 10^6 more (exactly one candidate agrees in every subinterval).
+This needs gcc and the C library the reference is built against (glibc 2.35
+here); the full table takes a few minutes on 8 cores.
 
     python sogrand_amd/csrc/make_log_table.py > sogrand_amd/csrc/sg_log_table.h
+    python sogrand_amd/csrc/make_log_table.py --rows 0,77,127   # a few rows only
 """
+import os
+import struct
+import subprocess
+import sys
+import tempfile
 from decimal import Decimal, getcontext
+from fractions import Fraction as Fr
+from multiprocessing import Pool
 
 getcontext().prec = 80
-LO, HI = -37, 53
-Q = Decimal(2) ** -42
+OFF = 0x3FE6000000000000
+N = 128
+LN2_HI = "0x1.62e42fefa3800p-1"
+LN2_LO = "0x1.ef35793c76730p-45"
+# log1p(r) - r ~ A0 r^2 + A1 r^3 + ... + A4 r^6 on the table path
+POLY = ["-0x1.0000000000001p-1", "0x1.555555551305bp-2", "-0x1.fffffffeb459p-3",
+        "0x1.999b324f10111p-3", "-0x1.55575e506c89fp-3"]
+# log(1 + r) ~ r + B0 r^2 + ... + B10 r^12 near x = 1 (B0 = -1/2)
+POLY1 = ["-0x1p-1", "0x1.5555555555577p-2", "-0x1.ffffffffffdcbp-3", "0x1.999999995dd0cp-3",
+         "-0x1.55555556745a7p-3", "0x1.24924a344de3p-3", "-0x1.fffffa4423d65p-4",
+         "0x1.c7184282ad6cap-4", "-0x1.999eb43b068ffp-4", "0x1.78182f7afd085p-4",
+         "-0x1.5521375d145cdp-4"]
+
+HELPER = r"""
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+typedef unsigned __int128 u128;
+static uint64_t au(double x) { uint64_t u; memcpy(&u, &x, 8); return u; }
+static double ad(uint64_t u) { double x; memcpy(&x, &u, 8); return x; }
+static u128 rd(const char *h, const char *l)
+{ return ((u128)strtoull(h, 0, 16) << 64) | strtoull(l, 0, 16); }
+static const double A[5] = {%(poly)s};
+/* the table path of the method for one (invc, logc), with the fused
+   multiply-adds of the C library's FMA build */
+static double tlog(double x, double invc, double logc)
+{
+    uint64_t ix = au(x), tmp = ix - 0x3fe6000000000000ull;
+    int k = (int)((int64_t)tmp >> 52);
+    double z = ad(ix - (tmp & (0xfffull << 52)));
+    double r = fma(z, invc, -1.0), kd = k;
+    double w = fma(kd, %(ln2hi)s, logc), hi = w + r;
+    double lo = fma(kd, %(ln2lo)s, (w - hi) + r), r2 = r * r;
+    double p = fma(r2, fma(r, A[4], A[3]), fma(r, A[2], A[1]));
+    return fma(r * r2, p, fma(r2, A[0], lo)) + hi;
+}
+static long agree(int i, double invc, double logc, long n, uint64_t s)
+{
+    double zlo = ad(0x3fe6000000000000ull + ((uint64_t)i << 45));
+    double zhi = ad(0x3fe6000000000000ull + ((uint64_t)(i + 1) << 45));
+    long bad = 0;
+    for (long j = 0; j < n; j++) {
+        s ^= s << 13; s ^= s >> 7; s ^= s << 17;
+        int k = (int)(j %% 9) - 4;
+        double x = ldexp(zlo + (double)(s >> 11) * 0x1p-53 * (zhi - zlo), k ? k : 5);
+        bad += au(tlog(x, invc, logc)) != au(log(x));
+    }
+    return bad;
+}
+int main(int argc, char **argv)
+{
+    if (argv[1][0] == 's') { /* scan: G d1 d2 d3 (hi lo each), j0, count */
+        u128 G = rd(argv[2], argv[3]), D1 = rd(argv[4], argv[5]);
+        u128 D2 = rd(argv[6], argv[7]), D3 = rd(argv[8], argv[9]);
+        long long j0 = atoll(argv[10]), cnt = atoll(argv[11]);
+        const u128 one = (u128)1 << 120, mask = one - 1, thr = (u128)1 << 100;
+        for (long long j = 0; j < cnt; j++) {
+            u128 f = G & mask;
+            if (f < thr || one - f < thr) printf("%%lld\n", j0 + j);
+            G += D1; D1 += D2; D2 += D3;
+        }
+        return 0;
+    }
+    /* pick: i, then candidate pairs on stdin; print the agreeing ones */
+    int i = atoi(argv[2]);
+    char a[64], b[64];
+    while (scanf("%%63s %%63s", a, b) == 2) {
+        double invc = strtod(a, 0), logc = strtod(b, 0);
+        if (agree(i, invc, logc, 50000, 0x9e3779b97f4a7c15ull ^ (uint64_t)i) == 0)
+            printf("%%a %%a %%ld\n", invc, logc, agree(i, invc, logc, 1000000, 12345 + i));
+    }
+    return 0;
+}
+"""
 
 
-def split(v: Decimal):
-    hi = (v / Q).to_integral_value() * Q
-    return float(hi), float(v - hi)
+def _d2u(x: float) -> int:
+    return struct.unpack("<Q", struct.pack("<d", x))[0]
+
+
+def _ulp(x: float) -> Fr:
+    return Fr(2) ** (((_d2u(x) >> 52) & 0x7FF) - 1023 - 52)
+
+
+def _fixed(q: Fr):
+    v = (q * (1 << 120)).__floor__() % (1 << 128)
+    return [f"{v >> 64:x}", f"{v & ((1 << 64) - 1):x}"]
+
+
+def candidates(exe: str, i: int, half: int = 1 << 29):
+    """Doubles invc within `half` ulps of 1/center_i for which log(1/invc) is
+    within 2^-66 of a multiple of 2^-43: (invc, logc) pairs."""
+    zlo = struct.unpack("<d", struct.pack("<Q", OFF + (i << 45)))[0]
+    zhi = struct.unpack("<d", struct.pack("<Q", OFF + ((i + 1) << 45)))[0]
+    v0 = 1.0 / ((zlo + zhi) / 2)
+    u, V0, S = _ulp(v0), Fr(v0), Fr(2) ** 43
+    t = u / V0
+    # -log(v0 + j u) 2^43 = A0 - S (j t - (j t)^2/2 + (j t)^3/3 - (j t)^4/4) + O(2^-50)
+    a0 = Fr(-Decimal(v0).ln() * Decimal(2) ** 43)
+    G = lambda j: a0 - S * (j * t - (j * t) ** 2 / 2 + (j * t) ** 3 / 3 - (j * t) ** 4 / 4)
+    g = [G(-half + k) for k in range(4)]
+    d = [g[1] - g[0], g[2] - 2 * g[1] + g[0], g[3] - 3 * g[2] + 3 * g[1] - g[0]]
+    args = [exe, "s", *_fixed(g[0]), *_fixed(d[0]), *_fixed(d[1]), *_fixed(d[2]),
+            str(-half), str(2 * half + 1)]
+    out = []
+    for js in subprocess.run(args, capture_output=True, text=True, check=True).stdout.split():
+        invc = float(V0 + int(js) * u)
+        L = (1 / Decimal(invc)).ln()
+        logc = float(L)
+        if (Fr(logc) * S).denominator == 1 and abs(Fr(L) - Fr(logc)) < Fr(2) ** -66:
+            out.append((invc, logc))
+    return out
+
+
+def row(args):
+    exe, i = args
+    cands = candidates(exe, i)
+    inp = "".join(f"{a.hex()} {b.hex()}\n" for a, b in cands)
+    res = subprocess.run([exe, "p", str(i)], input=inp, capture_output=True, text=True,
+                         check=True).stdout.split("\n")
+    res = [r.split() for r in res if r.strip()]
+    if len(res) != 1 or res[0][2] != "0":
+        raise SystemExit(f"subinterval {i}: {len(res)} candidates agree with the C library")
+    return i, float.fromhex(res[0][0]), float.fromhex(res[0][1]), len(cands)
+
+
+def build_helper(d: str) -> str:
+    src = os.path.join(d, "logtab.c")
+    exe = os.path.join(d, "logtab")
+    with open(src, "w") as f:
+        f.write(HELPER % {"poly": ", ".join(POLY), "ln2hi": LN2_HI, "ln2lo": LN2_LO})
+    subprocess.run(["gcc", "-O2", "-ffp-contract=off", src, "-o", exe, "-lm"], check=True)
+    return exe
 
 
 def main() -> None:
-    rows = []
-    for i in range(LO, HI + 1):
-        c = 1 + Decimal(i) / 128
-        invc = float(1 / c)
-        logc = -Decimal(invc).ln()
-        hi, lo = split(logc)
-        assert Decimal(hi) == (Decimal(hi) / Q).to_integral_value() * Q
-        rows.append(f"    {{{invc.hex()}, {hi.hex()}, {lo.hex()}}}, // c = 1 + {i}/128")
-    l2hi, l2lo = split(Decimal(2).ln())
-    body = "\n".join(rows)
-    print(f"""// Table of the device logarithm: for c = 1 + i/128, i = {LO} .. {HI}, invc = 1/c
-// rounded and -log(invc) = logc_hi + logc_lo (logc_hi a multiple of 2^-42),
-// generated with 80-digit decimal arithmetic by make_log_table.py.
+    only = None
+    if len(sys.argv) > 2 and sys.argv[1] == "--rows":
+        only = [int(x) for x in sys.argv[2].split(",")]
+    with tempfile.TemporaryDirectory() as d:
+        exe = build_helper(d)
+        idx = only if only is not None else list(range(N))
+        with Pool(min(8, os.cpu_count() or 1)) as p:
+            rows = p.map(row, [(exe, i) for i in idx])
+    lines = [f"    {{{a.hex()}, {b.hex()}}}, // {i}" for i, a, b, _ in rows]
+    if only is not None:
+        print("\n".join(lines))
+        return
+    body = "\n".join(lines)
+    print(f"""// Table of the decoder's log (the table-driven log of Arm Optimized Routines,
+// N = {N}): per subinterval i of z in [0x1.6p-1, 0x1.6p0), invc = 1/c and
+// logc = log(c) rounded, a multiple of 2^-43; the polynomial coefficients and
+// the ln 2 split of that design. Generated by make_log_table.py.
 #pragma once
 
 namespace sg {{
 
-constexpr int kLogTableFirst = {LO};
-constexpr int kLogTableEntries = {HI - LO + 1};
-constexpr double kLogLn2Hi = {l2hi.hex()}; // ln 2 rounded to a multiple of 2^-42
-constexpr double kLogLn2Lo = {l2lo.hex()}; // ln 2 - kLogLn2Hi
+constexpr int kLogTableBits = 7;
+constexpr double kLogLn2Hi = {LN2_HI}; // ln 2, high part (a multiple of 2^-42)
+constexpr double kLogLn2Lo = {LN2_LO}; // ln 2 - kLogLn2Hi
+// log1p(r) - r ~ r^2 (A0 + r A1 + ... + r^4 A4), |r| < 1/256 (table path)
+constexpr double kLogA[5] = {{{", ".join(POLY)}}};
+// log(1 + r) ~ r + r^2 B0 + ... + r^12 B10 (path near 1; B0 = -1/2)
+constexpr double kLogB[11] = {{
+    {", ".join(POLY1[:4])},
+    {", ".join(POLY1[4:8])},
+    {", ".join(POLY1[8:])}}};
 
 struct LogEntry {{
-    double invc, logc_hi, logc_lo;
+    double invc, logc;
 }};
 
-constexpr LogEntry kLogTable[{HI - LO + 1}] = {{
+constexpr LogEntry kLogTable[{N}] = {{
 {body}
 }};
 

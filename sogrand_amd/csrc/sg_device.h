@@ -10,8 +10,9 @@
 // that file unless another is named): every double is formed by the same
 // operations in the same order; the file is compiled with -ffp-contract=off so
 // nothing is fused. Where the reference calls the C library's exp and log,
-// the decoder uses its own table-driven ones (sg_exp.h, sg_log.h: within 0.54
-// and 0.502 ulp, as faithful as the library's). The policy PrecF64Legacy
+// the decoder uses table-driven ones with the library's own method, constants
+// and fused operations (sg_exp.h, sg_log.h), which give the library's results
+// bit for bit. The policy PrecF64Legacy
 // follows the reference's earlier SOGRAND_C/cubic_decoder.c instead.
 #pragma once
 
@@ -53,26 +54,20 @@ __device__ __forceinline__ void load_log_table(LogEntry *lds, int tid)
 }
 
 // exp(-pm) for pm >= 0, the probability weight of one guess (:626, :672, :712).
-// pm is capped at 746 (which also maps +inf there) to keep k in int range: at
-// the cap k >> 6 = -1077 and m < 2, so the result is 0.
+// Every lane forms the main path; the rare metrics from 512 on (and +inf) take
+// the library's large-argument branch, which runs only if some lane needs it.
+// (For -pm above -2^-54 the main path gives the library's 1 + x as well.)
 __device__ __forceinline__ double exp_neg(double pm, const ExpEntry *tab)
 {
-    // the cap as the instruction itself: as fmin the compiler first quiets a
-    // possible signalling NaN (one more instruction); pm is a sum of finite or
-    // infinite non-negative terms, never NaN
-    double pc;
-    asm("v_min_f64 %0, %1, %2" : "=v"(pc) : "v"(pm), "v"(746.0));
-    return exp_of(-pc, tab);
+    double y = exp_main(-pm, tab);
+    if (!(pm < 512.0)) y = exp_large(-pm, tab);
+    return y;
 }
 
-// exp(x) for any finite x: capped to [-746, 710], where the result is 0 or
-// overflows to +inf as exp's does (at 710, k >> 6 = 1024 and m > 1).
+// exp(x) for any x (the soft output's exp(llr), :547).
 __device__ __forceinline__ double exp_any(double x, const ExpEntry *tab)
 {
-    double xc;
-    asm("v_max_f64 %0, %1, %2" : "=v"(xc) : "v"(x), "v"(-746.0));
-    asm("v_min_f64 %0, %1, %2" : "=v"(xc) : "v"(xc), "v"(710.0));
-    return exp_of(xc, tab);
+    return exp_of(x, tab);
 }
 
 // Arithmetic of the per-guess path metrics.
@@ -139,7 +134,7 @@ __device__ __forceinline__ double jaclog(double x, double lim, const DecodeCfg &
 {
     if (x > lim) return x;
     if (x < -lim) return 0.0;
-    return log_pos(1.0 + exp_of(x, cfg.exptab), cfg.logtab);
+    return log_pos(1.0 + exp_main(x, cfg.exptab), cfg.logtab); // |x| <= lim < 512
 }
 
 

@@ -1,7 +1,10 @@
 // exp(x) of the line decoder (the guess weights exp(-pm), JacLog and the soft
-// output), table-driven after P. T. P. Tang (ACM TOMS 15(2), 1989). The same
-// source builds for the device (table in LDS) and for the host, where the CPU
-// tests measure it against the C library's exp.
+// output) with the arithmetic of the C library the reference calls: the
+// table-driven exp of Arm Optimized Routines (N = 128, degree-5 polynomial;
+// glibc's dbl-64 exp since 2.28), evaluated with the fused multiply-adds of that
+// library's x86-64 FMA build, so every result is the library's bit for bit.
+// The same source builds for the device (table in LDS) and for the host, where
+// the CPU tests compare it with the C library's exp.
 #pragma once
 
 #include <stdint.h>
@@ -12,53 +15,88 @@
 
 namespace sg {
 
-// One 16-byte entry of the LDS exp table. Entries 0..63 hold 2^(j/64) = a + b
-// (hi, lo); entries 64 and 65 hold the polynomial coefficients (c5, c4) and
-// (c3, c2), read at one address by every lane (a broadcast). Reading the
-// coefficients from LDS keeps them out of loop-invariant registers, which the
-// compiler otherwise pays for with a copy per step; as a separate entry rather
-// than a copy in every entry, the per-lane read at a data-dependent index
-// stays 16 bytes wide.
+// One 16-byte entry of the LDS exp table. Entries 0..127 hold T[j] and the
+// scale bits of 2^(j/128) (as a double's bits); entries 128 and 129 hold the
+// polynomial coefficients (C5, C4) and (C3, C2), read at one address by every
+// lane (a broadcast), which keeps them out of loop-invariant registers.
 struct ExpEntry {
     double a, b;
 };
-constexpr int kExpTableEntries = 66;
+constexpr int kExpTableEntries = 130;
 
 // Entry i of that table.
 __host__ __device__ constexpr ExpEntry exp_table_entry(int i)
 {
-    return i < 64 ? ExpEntry{kExp2Table[i].hi, kExp2Table[i].lo}
-           : i == 64 ? ExpEntry{kExpC5, kExpC4}
-                     : ExpEntry{kExpC3, kExpC2};
+    return i < 128 ? ExpEntry{kExp2Table[i].tail, __builtin_bit_cast(double, kExp2Table[i].sbits)}
+           : i == 128 ? ExpEntry{kExpC5, kExpC4}
+                      : ExpEntry{kExpC3, kExpC2};
 }
 
-// exp(x) for x in [-746, 710], table-driven after Tang: x = (64k + j) ln2/64 + r
-// with |r| <= ln2/128, exp(r) - 1 = q = r + r^2 (c2 + c3 r + c4 r^2 + c5 r^3)
-// (near-minimax, see make_exp_table.py), and 2^(j/64) = hi + lo from the LDS
-// table; the result hi + (hi q + lo (1 + q)) scaled by 2^k. The nearest
-// integer to x 64/ln2 comes from one FMA with 1.5 2^52, whose low word is then
-// that integer. Within 0.54 ulp; it differs from the correctly rounded value
-// on about 0.6 % of inputs. Results below 2^-1075 (x below about -745.13)
-// round to 0 in the final ldexp, as exp's do.
+// The reduction x = k ln2/128 + r (k from one FMA with the 1.5 2^52 shifter,
+// whose low word is then k), the table lookup and the polynomial:
+// exp(x) = scale (1 + tmp) with scale = 2^(k/128) as bits.
+struct ExpParts {
+    double tmp;
+    uint64_t sbits; // bits of 2^(k/128) (exponent may be out of range for |x| >= 512)
+    uint64_t ki;    // low word: k
+};
+
+__host__ __device__ inline ExpParts exp_parts(double x, const ExpEntry *tab)
+{
+    const double kd0 = __builtin_fma(x, kExpInvLn2N, kExpShift);
+    const uint64_t ki = __builtin_bit_cast(uint64_t, kd0);
+    const double kd = kd0 - kExpShift;
+    const double r = __builtin_fma(kd, kExpNegLn2LoN, __builtin_fma(kd, kExpNegLn2HiN, x));
+    const ExpEntry t = tab[ki & 127u];
+    const ExpEntry c54 = tab[128], c32 = tab[129];
+    const double r2 = r * r;
+    const double tmp = __builtin_fma(r2 * r2, __builtin_fma(r, c54.a, c54.b),
+                                     __builtin_fma(r2, __builtin_fma(r, c32.a, c32.b), t.a + r));
+    return {tmp, __builtin_bit_cast(uint64_t, t.b) + (ki << 45), ki};
+}
+
+// |x| < 512: scale + scale tmp, one fused multiply-add.
+__host__ __device__ inline double exp_main(double x, const ExpEntry *tab)
+{
+    const ExpParts p = exp_parts(x, tab);
+    const double scale = __builtin_bit_cast(double, p.sbits);
+    return __builtin_fma(scale, p.tmp, scale);
+}
+
+// |x| >= 512, infinities and NaN. For k > 0 the scale is formed 2^1009 lower
+// and multiplied back; for k < 0 it is formed 2^1022 higher, the sum is formed
+// unfused (the library's special case sits in its own block, where its
+// compiler does not contract), and a result below 2^-1022 is rounded once, at
+// its subnormal precision, before it is scaled down.
+__host__ __device__ inline double exp_large(double x, const ExpEntry *tab)
+{
+    if (!(__builtin_fabs(x) < 1024.0)) {
+        if (x != x) return 1.0 + x;
+        return x > 0.0 ? __builtin_inf() : 0.0;
+    }
+    const ExpParts p = exp_parts(x, tab);
+    if ((p.ki & 0x80000000u) == 0) {
+        const double scale = __builtin_bit_cast(double, p.sbits - (1009ull << 52));
+        return 0x1p1009 * __builtin_fma(scale, p.tmp, scale);
+    }
+    const double scale = __builtin_bit_cast(double, p.sbits + (1022ull << 52));
+    const double st = scale * p.tmp;
+    double y = scale + st;
+    if (y < 1.0) {
+        double lo = (scale - y) + st;
+        const double hi = 1.0 + y;
+        lo = ((1.0 - hi) + y) + lo;
+        y = (hi + lo) - 1.0;
+    }
+    return 0x1p-1022 * y;
+}
+
+// exp(x) for any x.
 __host__ __device__ inline double exp_of(double x, const ExpEntry *tab)
 {
-    const double shifter = 0x1.8p52;
-    const double tk = __builtin_fma(x, kExpInvLn2N, shifter);
-    const double kd = tk - shifter;
-    const int k = (int)(uint32_t)__builtin_bit_cast(uint64_t, tk);
-    const ExpEntry t = tab[k & 63];
-    const ExpEntry c54 = tab[64], c32 = tab[65];
-    double r = __builtin_fma(-kd, kExpLn2NHi, x);
-    r = __builtin_fma(-kd, kExpLn2NLo, r);
-    double q = c54.a;
-    q = __builtin_fma(q, r, c54.b);
-    q = __builtin_fma(q, r, c32.a);
-    q = __builtin_fma(q, r, c32.b);
-    q = __builtin_fma(q, r, 1.0);
-    q = q * r;
-    const double s = __builtin_fma(t.b, q, t.b);
-    const double m = __builtin_fma(t.a, q, s) + t.a;
-    return __builtin_ldexp(m, k >> 6);
+    double y = exp_main(x, tab);
+    if (!(__builtin_fabs(x) < 512.0)) y = exp_large(x, tab);
+    return y;
 }
 
 } // namespace sg

@@ -1,23 +1,20 @@
-// Natural logarithm for the line decoder's set-up (JacLog) and soft output,
-// table-driven after P. T. P. Tang (ACM TOMS 16(4), 1990). The same source
-// builds for the device (table in LDS) and for the host (the constant table),
-// where the CPU tests measure it against the C library's log.
+// Natural logarithm of the line decoder's set-up (JacLog) and soft output, with
+// the arithmetic of the C library the reference calls: the table-driven log
+// of Arm Optimized Routines (N = 128; glibc's dbl-64 log since 2.28), evaluated
+// with the fused multiply-adds of that library's x86-64 FMA build, so every
+// result is the library's bit for bit. The same source builds for the device
+// (table in LDS) and for the host, where the CPU tests compare it with the C
+// library's log.
 //
-// x = 2^k m with m in [sqrt(1/2), sqrt(2)); c = 1 + i/128 is the table point
-// nearest m, invc = 1/c rounded and logc = -log(invc) (so log(m) =
-// logc + log(m invc) exactly). m invc = p_hi + p_lo exactly (a product and
-// its FMA residual) and r = p_hi - 1 is exact (Sterbenz), |r| < 0.0056, so
-//   log(x) = k ln2 + logc + r + p_lo (1 - r) + (log1p(r) - r)
-// with log1p(r) - r by its Taylor series to r^8 (truncation below 2^-62 of
-// the result). k ln2_hi + logc_hi is exact (both multiples of 2^-42), its
-// sum with r is split into s + err exactly (Fast2Sum: |hi| >= |r| or hi = 0),
-// and the small terms are added before the one final rounding s + lo.
-// Defined for positive normal finite x, the only arguments the decoder forms
-// (1 + exp(v) with |v| <= 50, and probability sums floored at 1e-40 or
-// more); the range is not checked, which keeps the code small.
+// Table path: x = 2^k z with z in [0x1.6p-1, 0x1.6p0) (exact), subinterval i
+// from the top 7 bits of z's offset, r = z invc - 1 by one FMA, and
+//   log(x) = (k ln2_hi + logc) + r + k ln2_lo + r^2 A0 + r^3 (A1 + ... + r^3 A4)
+// with the first sum split exactly into hi + lo. Near 1 (x - 1 in
+// [-2^-4, 0x1.09p-4)) a polynomial in r = x - 1 with the r^2/2 term split so
+// that it is exact.
 #pragma once
 
-#include <math.h>
+#include <stdint.h>
 
 #include "sg_log_table.h"
 
@@ -25,35 +22,61 @@
 
 namespace sg {
 
+constexpr int kLogTableEntries = 128;
+constexpr uint64_t kLogNear1Lo = 0x3fee000000000000ull; // bits of 1 - 2^-4
+constexpr uint64_t kLogNear1Hi = 0x3ff1090000000000ull; // bits of 1 + 0x1.09p-4
+
+__host__ __device__ inline double log_near1(double x)
+{
+    const double r = x - 1.0, r2 = r * r, r3 = r * r2;
+    const double q3 = __builtin_fma(r3, kLogB[10], __builtin_fma(r2, kLogB[9], __builtin_fma(r, kLogB[8], kLogB[7])));
+    const double q2 = __builtin_fma(r3, q3, __builtin_fma(r2, kLogB[6], __builtin_fma(r, kLogB[5], kLogB[4])));
+    const double q1 = __builtin_fma(r3, q2, __builtin_fma(r2, kLogB[3], __builtin_fma(r, kLogB[2], kLogB[1])));
+    // r = rhi + rlo with rhi of 26 bits, so rhi^2 B0 is exact
+    const double w27 = r * 0x1p27;
+    const double rhi = (r + w27) - w27, rlo = r - rhi;
+    const double w = rhi * rhi * kLogB[0];
+    const double hi = r + w;
+    double lo = (r - hi) + w;
+    lo = __builtin_fma(kLogB[0] * rlo, rhi + r, lo);
+    const double y = __builtin_fma(r3, q1, lo);
+    return y + hi;
+}
+
+// Table path for the bits ix of a positive normal x.
+__host__ __device__ inline double log_table(uint64_t ix, const LogEntry *tab)
+{
+    const uint64_t tmp = ix - 0x3fe6000000000000ull;
+    const int i = (int)((tmp >> (52 - kLogTableBits)) & 127u);
+    const int k = (int)((int64_t)tmp >> 52);
+    const double z = __builtin_bit_cast(double, ix - (tmp & (0xfffull << 52)));
+    const LogEntry c = tab[i];
+    const double r = __builtin_fma(z, c.invc, -1.0);
+    const double kd = (double)k;
+    const double w = __builtin_fma(kd, kLogLn2Hi, c.logc); // exact
+    const double hi = w + r;
+    const double lo = __builtin_fma(kd, kLogLn2Lo, (w - hi) + r);
+    const double r2 = r * r;
+    const double p = __builtin_fma(r2, __builtin_fma(r, kLogA[4], kLogA[3]),
+                                   __builtin_fma(r, kLogA[2], kLogA[1]));
+    return __builtin_fma(r * r2, p, __builtin_fma(r2, kLogA[0], lo)) + hi;
+}
+
+// log(x) for any x (the decoder forms only positive normal arguments: 1 +
+// exp(v) and probability sums floored at EPSILON; zero, subnormals,
+// negatives, infinities and NaN follow the library too).
 __host__ __device__ inline double log_pos(double x, const LogEntry *tab)
 {
-    int e;
-    const double mant = frexp(x, &e); // [0.5, 1)
-    const bool low = mant < 0x1.6a09e667f3bcdp-1; // sqrt(1/2), rounded up
-    const double m = low ? 2.0 * mant : mant;   // [sqrt(1/2), sqrt(2)), exact
-    const int k = low ? e - 1 : e;
-    const int i = (int)rint(fma(m, 128.0, -128.0)); // (m - 1) 128 is exact
-    const LogEntry c = tab[i - kLogTableFirst];
-    const double ph = m * c.invc;
-    const double pl = fma(m, c.invc, -ph);
-    const double r = ph - 1.0;
-    const double r2 = r * r;
-    double q = -0x1p-3;                       // -1/8
-    q = fma(q, r, 0x1.2492492492492p-3);      // 1/7
-    q = fma(q, r, -0x1.5555555555555p-3);     // -1/6
-    q = fma(q, r, 0x1.999999999999ap-3);      // 1/5
-    q = fma(q, r, -0x1p-2);                   // -1/4
-    q = fma(q, r, 0x1.5555555555555p-2);      // 1/3
-    q = fma(q, r, -0x1p-1);                   // -1/2
-    q = q * r2;                               // log1p(r) - r
-    const double kd = (double)k;
-    const double hi = fma(kd, kLogLn2Hi, c.logc_hi); // exact
-    const double s = hi + r;
-    const double err = (hi - s) + r;
-    double lo = fma(kd, kLogLn2Lo, c.logc_lo) + err;
-    lo = lo + fma(-r, pl, pl);
-    lo = lo + q;
-    return s + lo;
+    uint64_t ix = __builtin_bit_cast(uint64_t, x);
+    if (ix - kLogNear1Lo < kLogNear1Hi - kLogNear1Lo) return log_near1(x);
+    const uint32_t top = (uint32_t)(ix >> 48);
+    if (top - 0x0010u >= 0x7ff0u - 0x0010u) {
+        if ((ix << 1) == 0) return -__builtin_inf();
+        if (ix == 0x7ff0000000000000ull) return x;
+        if ((top & 0x8000u) || (top & 0x7ff0u) == 0x7ff0u) return (x - x) / (x - x);
+        ix = __builtin_bit_cast(uint64_t, x * 0x1p52) - (52ull << 52); // subnormal
+    }
+    return log_table(ix, tab);
 }
 
 } // namespace sg

@@ -22,3 +22,50 @@ def test_bench_help_lists_the_workloads():
     assert r.returncode == 0
     for w in ("headline", "snr-sweep", "uneven", "mixed-precision", "serving", "variants", "copy"):
         assert w in r.stdout
+
+
+def _bench():
+    sys.path.insert(0, ROOT)
+    import importlib
+
+    return importlib.import_module("bench")
+
+
+def test_serving_leg_refuses_a_non_persistent_decoder():
+    # the serving leg must measure the batched kernel; a decoder left on the
+    # plain backend (as after the plain timing) is refused before any GPU work
+    import sogrand_amd as sg
+
+    bench = _bench()
+
+    class FakeDecoder:
+        class params:
+            backend = sg.BACKEND_PLAIN
+
+    import pytest
+
+    with pytest.raises(ValueError):
+        bench.serving_capacity(FakeDecoder(), 1)
+    with pytest.raises(ValueError):
+        bench.serve_load(FakeDecoder(), 1000.0, 0.5, 1.0, 1)
+
+
+def test_traffic_record_matches_the_sources_or_is_refused():
+    # the committed counter record carries the hash of the sources it was
+    # measured on; bench.py reports its traffic only for that build
+    import json
+
+    bench = _bench()
+    path = os.path.join(ROOT, "profiles", "persistent_100k_cw_2dB_traffic.json")
+    rec = json.load(open(path))
+    val, note = bench.measured_traffic(rec["n_cw"], rec["ebn0_db"])
+    if rec.get("source_sha256") == bench.kernel_source_hash():
+        assert val == round(rec["traffic_bytes_per_launch"])
+    else:
+        assert val is None and "stale" in note
+    assert bench.measured_traffic(rec["n_cw"] + 1, rec["ebn0_db"])[0] is None
+
+
+def test_cpu_baseline_uses_every_allowed_core():
+    bench = _bench()
+    assert bench.host_cores() == len(os.sched_getaffinity(0))

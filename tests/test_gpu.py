@@ -1,18 +1,15 @@
 """Parity of the gfx950 decoder (through the C ABI) with the reference and the oracle.
 
-The bar, and why:
-- decoded bits and half-iteration counts: bit-exact on every codeword;
-- guess counts NG and NG_p: bit-exact on every codeword whose counts the
-  reference's arithmetic fixes, i.e. where its flow build and its FMA build
-  (same source, -O2 -march=native) agree - at 1 dB about 1 codeword in 100 gets
-  different counts from the two reference builds, and those are not compared;
-- soft outputs of one component decode: within 1e-9 relative (1e-9 absolute
-  floor) plus 4 kappa, kappa = the rounding-error bound of the reference's own
-  1 - found-mass term (chain_cond). GPU exp/log differ from glibc's by an ulp
-  and that term is a long cancelling subtraction chain;
-- soft outputs of whole decodes: at least as close to the reference as the
-  reference's FMA build is (fraction within 1e-9 and within 1e-6, 3 points of
-  slack), and within 1e-9 wherever the last line decode is well conditioned.
+The bar, on every line and every codeword tested:
+- decoded bits, half-iteration counts, guess counts NG and NG_p and the
+  convergence flag: bit-exact;
+- soft outputs (component L_APP / L_E and final L_APP): within 1e-9 relative
+  (1e-9 absolute floor, SOFT_RTOL in conftest.py).
+The decoder's exp and log give the C library's results bit for bit (checked on
+the CPU by tests/log_check.cpp), and it forms every other double with the
+reference's operations in the reference's order, so nothing here is exempted:
+not even the codewords on which the reference's own FMA build disagrees with
+its flow build.
 """
 import json
 import os
@@ -22,8 +19,7 @@ import sys
 import numpy as np
 import pytest
 
-from conftest import (GOLDEN, ROOT, chain_cond, frac_within, golden, soft_close, soft_err,
-                      stable_codewords, within_chain_bound)
+from conftest import GOLDEN, ROOT, frac_within, golden, soft_close, soft_err, stable_codewords
 
 pytestmark = pytest.mark.gpu
 
@@ -49,13 +45,10 @@ def test_lines_match_reference(decoder):
     _say("lines:", g["llr"].shape)
     la, le, ng = decoder.bitso(g["llr"])
     assert (ng == g["N_guess"]).all(), np.flatnonzero(ng != g["N_guess"])[:10]
-    kappa = chain_cond(g["T"], g["pNL"], g["curL"])[:, None]
-    assert within_chain_bound(la, g["L_APP"], kappa).all()
-    assert within_chain_bound(le, g["L_E"], kappa).all()
-    # well-conditioned lines meet the plain 1e-9 bar
-    good = (kappa[:, 0] <= 1e-11)
-    assert good.sum() > 0.4 * len(good)
-    assert soft_close(la[good], g["L_APP"][good]), soft_err(la[good], g["L_APP"][good])
+    assert soft_close(la, g["L_APP"]), soft_err(la, g["L_APP"])
+    assert soft_close(le, g["L_E"]), soft_err(le, g["L_E"])
+    _say("  lines bit-identical to the reference:",
+         int((la == g["L_APP"]).all(axis=1).sum()), "of", len(la))
 
 
 def test_lines_random_match_oracle(decoder, oracle):
@@ -66,9 +59,8 @@ def test_lines_random_match_oracle(decoder, oracle):
     for i in range(len(x)):
         la2, le2, tr = oracle.siso(x[i])
         assert ng[i] == tr.T, i
-        k = oracle.chain_cond(tr)
-        assert within_chain_bound(la[i], la2, k).all(), i
-        assert within_chain_bound(le[i], le2, k).all(), i
+        assert soft_close(la[i], la2), (i, soft_err(la[i], la2))
+        assert soft_close(le[i], le2), (i, soft_err(le[i], le2))
 
 
 @pytest.mark.parametrize("backend", [0, 1], ids=["plain", "persistent"])
@@ -82,57 +74,40 @@ def test_cubes_match_reference_binary(decoder, oracle, backend, snr):
     llr = sg.gen_llr(m["seed"], m["n"], ebn0_db=float(snr))
     r = decoder.decode(llr, soft=True)
     assert (r.bits == g["bits"]).all()
-    assert (r.stats["half_iters"] == g["half_iters"]).all()
+    for f in ("half_iters", "NG", "NG_p"):
+        bad = np.flatnonzero(r.stats[f] != g[f])
+        assert len(bad) == 0, (f, bad[:10])
     ref = np.zeros(m["n"], sg.CW_STATS_DTYPE)
     fma = np.zeros(m["n"], sg.CW_STATS_DTYPE)
-    for f, k in (("half_iters", "half_iters"), ("NG", "NG"), ("NG_p", "NG_p")):
-        ref[f] = g[k]
-        fma[f] = g[k + "_fma"]
-    stable = stable_codewords(ref, fma, g["bits"], g["bits_fma"])
-    _say("  codewords the reference builds agree on:", int(stable.sum()), "of", m["n"])
-    assert stable.mean() >= 0.97
-    assert (r.stats["NG"][stable] == g["NG"][stable]).all()
-    assert (r.stats["NG_p"][stable] == g["NG_p"][stable]).all()
-    # soft outputs: as close to the reference as the reference's FMA build is
+    for f in ("half_iters", "NG", "NG_p"):
+        ref[f] = g[f]
+        fma[f] = g[f + "_fma"]
+    unstable = ~stable_codewords(ref, fma, g["bits"], g["bits_fma"])
+    _say("  exact on all", m["n"], "codewords, including", int(unstable.sum()),
+         "on which the reference's FMA build differs from its flow build")
+    # final soft outputs: the reference's (recorded for the first codewords)
     ns = g["soft"].shape[0]
-    f_gpu, f_fma = frac_within(r.soft[:ns], g["soft"]), frac_within(g["soft_fma"], g["soft"])
-    f6_gpu = frac_within(r.soft[:ns], g["soft"], 1e-6)
-    f6_fma = frac_within(g["soft_fma"], g["soft"], 1e-6)
-    _say(f"  soft within 1e-9: gpu {f_gpu:.3f} ref-fma {f_fma:.3f}; within 1e-6: gpu {f6_gpu:.3f} "
-         f"ref-fma {f6_fma:.3f}")
-    assert f_gpu >= f_fma - 0.03 and f6_gpu >= f6_fma - 0.03
-    _, _, soft_o, cond = oracle.decode(llr[:ns], soft=True, cond=True)
-    well = cond <= 1e-11
-    assert soft_close(r.soft[:ns][well], soft_o[well])
+    assert soft_close(r.soft[:ns], g["soft"]), soft_err(r.soft[:ns], g["soft"])
+    _say(f"  soft: within 1e-9 everywhere (max {soft_err(r.soft[:ns], g['soft']):.2e}); "
+         f"the reference's FMA build: {frac_within(g['soft_fma'], g['soft']):.3f} within 1e-9")
 
 
 @pytest.mark.parametrize("backend", [0, 1], ids=["plain", "persistent"])
-def test_cubes_2000_match_oracle(decoder, oracle, oracle_fma, backend):
+def test_cubes_2000_match_oracle(decoder, oracle, backend):
     # the 2,000-codeword subset at 2 dB, checked against the oracle
     sg = _sg()
     llr = sg.gen_llr(4242, 2000, ebn0_db=2.0)
     decoder.set_params(backend=backend)
     r = decoder.decode(llr, soft=True)
-    bits, st, soft, cond = oracle.decode(llr, soft=True, cond=True)
-    bits_f, st_f, soft_f = oracle_fma.decode(llr, soft=True)
-    stable = stable_codewords(st, st_f, bits, bits_f)
-    _say("2000 cw: oracle builds agree on", int(stable.sum()))
-    assert stable.mean() >= 0.99
+    bits, st, soft = oracle.decode(llr, soft=True)
     assert (r.bits == bits).all()
-    assert (r.stats["half_iters"] == st["half_iters"]).all()
-    for f in ("NG", "NG_p", "converged"):
-        assert (r.stats[f][stable] == st[f][stable]).all(), f
-    well = cond <= 1e-11
-    assert soft_close(r.soft[well], soft[well])
-    # the rest: as close to the oracle as its own FMA build is
-    for tol in (1e-9, 1e-6):
-        fg, ff = frac_within(r.soft, soft, tol), frac_within(soft_f, soft, tol)
-        _say(f"  within {tol:g}: gpu {fg:.3f} oracle-fma {ff:.3f}")
-        assert fg >= ff - 0.03
+    for f in ("half_iters", "NG", "NG_p", "converged"):
+        assert (r.stats[f] == st[f]).all(), f
+    assert soft_close(r.soft, soft), soft_err(r.soft, soft)
 
 
 @pytest.mark.parametrize("backend", [0, 1], ids=["plain", "persistent"])
-def test_cubes_with_ties_match_oracle(decoder, oracle, oracle_fma, backend):
+def test_cubes_with_ties_match_oracle(decoder, oracle, backend):
     # channel LLRs on a grid of 1/4: most lines start with equal magnitudes, so
     # the set-up's tie detection and the quicksort replay run in every kernel
     sg = _sg()
@@ -142,15 +117,10 @@ def test_cubes_with_ties_match_oracle(decoder, oracle, oracle_fma, backend):
     assert ties.mean() > 0.9
     decoder.set_params(backend=backend)
     r = decoder.decode(llr, soft=True)
-    bits, st, soft, cond = oracle.decode(llr, soft=True, cond=True)
-    bits_f, st_f, _ = oracle_fma.decode(llr)
-    stable = stable_codewords(st, st_f, bits, bits_f)
-    _say("ties: oracle builds agree on", int(stable.sum()), "of", len(llr))
-    assert stable.mean() >= 0.9
-    assert (r.bits[stable] == bits[stable]).all()
-    assert (r.stats[stable] == st[stable]).all()
-    well = cond <= 1e-11
-    assert soft_close(r.soft[well], soft[well])
+    bits, st, soft = oracle.decode(llr, soft=True)
+    assert (r.bits == bits).all()
+    assert (r.stats == st).all()
+    assert soft_close(r.soft, soft), soft_err(r.soft, soft)
 
 
 def test_backends_identical_and_batch_independent(decoder):
@@ -173,20 +143,19 @@ def test_backends_identical_and_batch_independent(decoder):
 
 
 @pytest.mark.parametrize("tmax", [1000, 100000])
-def test_finite_tmax_matches_oracle(decoder, oracle, oracle_fma, tmax):
+def test_finite_tmax_matches_oracle(decoder, oracle, tmax):
+    # the uneven configuration's budgets, per-codeword Eb/N0 over its whole
+    # 0.5-3.5 dB range
     sg = _sg()
-    llr = sg.gen_llr(31337, 150, ebn0_db=np.linspace(1.5, 3.5, 150))
-    bits, st, soft, cond = oracle.decode(llr, Tmax=tmax, soft=True, cond=True)
-    bits_f, st_f, _ = oracle_fma.decode(llr, Tmax=tmax)
-    stable = stable_codewords(st, st_f, bits, bits_f)
-    assert stable.mean() >= 0.95
+    ebn0 = np.random.default_rng(tmax).uniform(0.5, 3.5, 150)
+    llr = sg.gen_llr(31337, 150, ebn0_db=ebn0)
+    bits, st, soft = oracle.decode(llr, Tmax=tmax, soft=True)
     for backend in (0, 1):
         decoder.set_params(backend=backend, Tmax=tmax)
         r = decoder.decode(llr, soft=True)
-        assert (r.bits[stable] == bits[stable]).all()
-        assert (r.stats[stable] == st[stable]).all()
-        well = cond <= 1e-11
-        assert soft_close(r.soft[well], soft[well])
+        assert (r.bits == bits).all()
+        assert (r.stats == st).all()
+        assert soft_close(r.soft, soft), soft_err(r.soft, soft)
     decoder.set_params(Tmax=sg.UINT64_MAX)
     if tmax >= 65536:  # at most 65536 guesses per line: a budget beyond it never binds
         r2 = decoder.decode(llr)
@@ -388,9 +357,8 @@ def test_flow_script_end_to_end(tmp_path):
 
 @pytest.mark.parametrize("backend", [0, 1], ids=["plain", "persistent"])
 def test_epsilon_variant_matches_reference_variant(decoder, oracle, backend):
-    # the reference's DBL_EPSILON variant (cubic_decoder_updated.c): lines within
-    # the chain bound, whole decodes bit-exact on bits and half-iterations and on
-    # the counts its two builds agree on
+    # the reference's DBL_EPSILON variant (cubic_decoder_updated.c): lines and
+    # whole decodes to the same bar as the final decoder
     sg = _sg()
     from oracle import DBL_EPSILON
 
@@ -400,24 +368,16 @@ def test_epsilon_variant_matches_reference_variant(decoder, oracle, backend):
     try:
         la, le, ng = decoder.bitso(g["llr"])
         assert (ng == g["N_guess"]).all()
-        kappa = chain_cond(g["T"], g["pNL"], g["curL"])[:, None]
-        assert within_chain_bound(la, g["L_APP"], kappa).all()
-        assert within_chain_bound(le, g["L_E"], kappa).all()
+        assert soft_close(la, g["L_APP"]), soft_err(la, g["L_APP"])
+        assert soft_close(le, g["L_E"]), soft_err(le, g["L_E"])
         for tag, m in meta["cubes"].items():
             llr = sg.gen_llr(m["seed"], m["n"], ebn0_db=float(tag))
-            r = decoder.decode(llr)
+            r = decoder.decode(llr, soft=True)
             assert (r.bits == g[f"bits_{tag}"]).all()
-            assert (r.stats["half_iters"] == g[f"half_iters_{tag}"]).all()
-            ref = np.zeros(m["n"], sg.CW_STATS_DTYPE)
-            fma = np.zeros(m["n"], sg.CW_STATS_DTYPE)
             for f in ("half_iters", "NG", "NG_p"):
-                ref[f] = g[f"{f}_{tag}"]
-                fma[f] = g[f"{f}_fma_{tag}"]
-            stable = stable_codewords(ref, fma, g[f"bits_{tag}"], g[f"bits_fma_{tag}"])
-            _say(f"variant {tag} dB: builds agree on {int(stable.sum())} of {m['n']}")
-            assert stable.mean() >= 0.95
-            assert (r.stats["NG"][stable] == g[f"NG_{tag}"][stable]).all()
-            assert (r.stats["NG_p"][stable] == g[f"NG_p_{tag}"][stable]).all()
+                assert (r.stats[f] == g[f"{f}_{tag}"]).all(), (tag, f)
+            ref = g[f"soft_{tag}"]
+            assert soft_close(r.soft[:ref.shape[0]], ref), soft_err(r.soft[:ref.shape[0]], ref)
     finally:
         decoder.set_params(epsilon=1e-40)
 
