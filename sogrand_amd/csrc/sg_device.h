@@ -33,24 +33,24 @@ struct DecodeCfg {
     const uint16_t *sched;  // schedule buffer: kSchedTables x kSchedStride masks, then
                             // the position words (pos_tables, sg_common.h)
     uint32_t L;             // list size 1..kMaxL
-    const ExpEntry *exptab; // exp table in LDS (load_exp_table)
-    const LogEntry *logtab; // log table in LDS (load_log_table)
+    const ExpEntry *exptab; // exp table in LDS, then {eps, thres} (load_exp_table)
+    const LogEntry *logtab; // log table (kLogTable, in global memory: read only by
+                            // the set-up and the soft output, it stays in the caches)
     double eps;             // EPSILON of the confidence test, P_notinlist floor and clamps
     double eps_log;         // floor inside the soft output's logs (eps, or 1e-30 for
                             // cubic_decoder.c:533)
 };
 
-// Fill the LDS exp table (first kExpTableEntries threads) - the caller synchronises.
-__device__ __forceinline__ void load_exp_table(ExpEntry *lds, int tid)
+// Entries of the LDS table: the exp table, then one entry {eps, thres} that the
+// list insertion reads where it needs them (as a broadcast load, which keeps
+// the two doubles out of the scalar registers the search loop runs short of).
+constexpr int kLdsTableEntries = kExpTableEntries + 1;
+
+// Fill the LDS table (first kLdsTableEntries threads) - the caller synchronises.
+__device__ __forceinline__ void load_exp_table(ExpEntry *lds, int tid, double eps, double thres)
 {
     if (tid < kExpTableEntries) lds[tid] = exp_table_entry(tid);
-}
-
-// Fill the LDS log table (first kLogTableEntries threads) - the caller
-// synchronises.
-__device__ __forceinline__ void load_log_table(LogEntry *lds, int tid)
-{
-    if (tid < kLogTableEntries) lds[tid] = kLogTable[tid];
+    else if (tid == kExpTableEntries) lds[tid] = ExpEntry{eps, thres};
 }
 
 // exp(-pm) for pm >= 0, the probability weight of one guess (:626, :672, :712).
@@ -62,6 +62,29 @@ __device__ __forceinline__ double exp_neg(double pm, const ExpEntry *tab)
     double y = exp_main(-pm, tab);
     if (!(pm < 512.0)) y = exp_large(-pm, tab);
     return y;
+}
+
+// exp(-pm) of a guess in the search loop: the main path, with pm capped at
+// 708 (one v_min_f64) so that the scale stays a normal double. Exact for
+// pm < 512. A larger metric gets a stand-in in (0, 2^-738] instead of its
+// weight (also below 2^-738): subtracting either from a P_notGuess of magnitude
+// 2^-680 or more leaves it unchanged (both are below half its ulp), and since
+// P_notGuess never increases, a search that ends with P_notGuess >= 2^-680
+// has formed the exact chain. A hit with such a metric (whose weight would
+// enter the found mass) poisons P_notGuess with NaN, which ends the search at
+// once; a search that ends with P_notGuess below 2^-680 or NaN is then run
+// again with exact weights (chain_exact, search_exact).
+__device__ __forceinline__ double exp_neg_loop(double pm, const ExpEntry *tab)
+{
+    double pc;
+    asm("v_min_f64 %0, %1, %2" : "=v"(pc) : "v"(pm), "v"(708.0));
+    return exp_main(-pc, tab);
+}
+
+// Whether a finished search formed its chain exactly (see exp_neg_loop).
+__device__ __forceinline__ bool chain_exact(double pnot)
+{
+    return pnot >= 0x1p-680;
 }
 
 // exp(x) for any x (the soft output's exp(llr), :547).
@@ -420,10 +443,11 @@ __device__ __forceinline__ bool list_hit(LineList &ls, double e, uint32_t code, 
     ls.ppos += e;
     const double pneg = 0.00390625 * pnot; // 2^-(n-k) * P_notGuess
     ls.pnl = pneg;
-    const double conf = (ls.ppos + pneg > cfg.eps) ? ls.ppos / (ls.ppos + pneg) : 1.0;
+    const ExpEntry et = cfg.exptab[kExpTableEntries]; // {eps, thres}
+    const double conf = (ls.ppos + pneg > et.a) ? ls.ppos / (ls.ppos + pneg) : 1.0;
     ls.code |= (uint64_t)code << (16 * ls.nl);
     ls.nl += 1;
-    return conf > cfg.thres || (uint32_t)ls.nl == cfg.L;
+    return conf > et.b || (uint32_t)ls.nl == cfg.L;
 }
 
 // Entry q (< ls.nl) of a line's list, rebuilt from its schedule position with
@@ -485,6 +509,8 @@ struct SearchState {
     uint32_t t;       // next schedule index; the guesses so far are t + 1 (the
                       // hard decision and t patterns)
     double pnot;      // P_notGuess (:614, :626, :672, :712)
+    uint32_t pm_hi;   // largest high word of the loop's path metrics (>= 0): at
+                      // least 0x40800000 iff some metric reached 512
     bool done;
     const uint4 *tp;  // position-word table of the line's intercept
     uint4 nxt;        // the next chunk of 8 position words (prefetched):
@@ -500,9 +526,10 @@ __device__ __forceinline__ const uint32_t *pos_tables(const uint16_t *sched)
 }
 
 // Hard-decision guess (:624-641) and schedule selection.
+// The hard-decision guess alone (:624-641): list, P_notGuess, done, t.
 template <class P>
-__device__ __forceinline__ void search_init(SearchState &ss, LineList &ls, uint32_t hd, uint32_t s0,
-                                            double pm_hd, int32_t ic, const DecodeCfg &cfg)
+__device__ __forceinline__ void search_start(SearchState &ss, LineList &ls, uint32_t s0,
+                                             double pm_hd, const DecodeCfg &cfg)
 {
     ls.nl = 0;
     ls.pnl = 0.0;
@@ -519,11 +546,19 @@ __device__ __forceinline__ void search_init(SearchState &ss, LineList &ls, uint3
     }
     ss.done = false;
     if (s0 == 0) ss.done = list_hit(ls, e_hd, 0u, ss.pnot, cfg);
+    ss.t = 0;
+    ss.pm_hi = 0;
+}
+
+template <class P>
+__device__ __forceinline__ void search_init(SearchState &ss, LineList &ls, uint32_t hd, uint32_t s0,
+                                            double pm_hd, int32_t ic, const DecodeCfg &cfg)
+{
+    search_start<P>(ss, ls, s0, pm_hd, cfg);
     const int tbl = ic < kSchedGenericIc ? ic : kSchedGenericIc;
     ss.tp = reinterpret_cast<const uint4 *>(pos_tables(cfg.sched) + (size_t)tbl * kSchedStride);
     ss.nxt = ss.tp[0];
     ss.nxt2 = ss.tp[1];
-    ss.t = 0;
 }
 
 // Accounting of a pair of guesses at schedule positions t and t + 1 from their
@@ -547,9 +582,21 @@ __device__ __forceinline__ void pair_account(SearchState &ss, LineList &ls, cons
         // (rare) hit branch below
         e1 = exp_neg_f32((float)pm1);
         e2 = exp_neg_f32((float)pm2);
-    } else if constexpr (kPmLoop) {
+    } else if constexpr (kPmLoop && P::kLegacy) {
+        // exact weights: the earlier decoder's P_neg is a guess's own weight
         e1 = exp_neg((double)pm1, cfg.exptab);
         e2 = exp_neg((double)pm2, cfg.exptab);
+    } else if constexpr (kPmLoop) {
+        e1 = exp_neg_loop((double)pm1, cfg.exptab);
+        e2 = exp_neg_loop((double)pm2, cfg.exptab);
+    }
+    constexpr bool kStandIn = kPmLoop && !P::kLegacy && !P::kF32;
+    if constexpr (kStandIn) {
+        // one v_max3_u32 per pair: metrics are >= 0, so their high words order
+        // like their values
+        const uint32_t h1 = (uint32_t)(__builtin_bit_cast(uint64_t, (double)pm1) >> 32);
+        const uint32_t h2 = (uint32_t)(__builtin_bit_cast(uint64_t, (double)pm2) >> 32);
+        ss.pm_hi = max(ss.pm_hi, max(h1, h2));
     }
     ss.t = t + 1;
     if (P::kF32 && hit1) e1 = exp_neg((double)pm1, cfg.exptab);
@@ -558,7 +605,12 @@ __device__ __forceinline__ void pair_account(SearchState &ss, LineList &ls, cons
         ss.pnot -= e1;
     else if (kPmLoop && !hit1)
         ls.pnl = e1; // cubic_decoder.c:638
-    bool stop = hit1 && list_hit(ls, e1, t + 1u, ss.pnot, cfg);
+    bool stop = false;
+    if (hit1) {
+        if constexpr (kStandIn)
+            if (!((double)pm1 < 512.0)) ss.pnot = __builtin_nan("");
+        stop = list_hit(ls, e1, t + 1u, ss.pnot, cfg);
+    }
     // end of the schedule, or the next weight level is over budget
     if (!stop && !take2) stop = true;
     if (!stop) {
@@ -569,7 +621,11 @@ __device__ __forceinline__ void pair_account(SearchState &ss, LineList &ls, cons
             ss.pnot -= e2;
         else if (kPmLoop && !hit2)
             ls.pnl = e2;
-        stop = hit2 && list_hit(ls, e2, t + 2u, ss.pnot, cfg);
+        if (hit2) {
+            if constexpr (kStandIn)
+                if (!((double)pm2 < 512.0)) ss.pnot = __builtin_nan("");
+            stop = list_hit(ls, e2, t + 2u, ss.pnot, cfg);
+        }
     }
     ss.done = stop;
 }
@@ -674,6 +730,44 @@ __device__ __forceinline__ void search_pair(SearchState &ss, LineList &ls, const
                              [&](int j) { return (uint32_t)masks[t + (uint32_t)j]; });
 }
 
+// The search of one line one guess at a time with exact weights (the same
+// schedule, sums, budget guard and stopping rule as search_pair and
+// pair_account, :624-744), for the rare searches whose P_notGuess chain needs
+// them (chain_exact). Rolled loops: small code, few registers; it leaves the
+// search loop's prefetch registers (ss.nxt, ss.nxt2), whose loads may still be
+// in flight, alone.
+template <bool FINITE_T, class P, class LDS>
+__device__ __forceinline__ void search_exact(SearchState &ss, LineList &ls, const LDS &S,
+                                             uint32_t hd, uint32_t s0, double pm_hd, int32_t ic,
+                                             const DecodeCfg &cfg)
+{
+    search_start<P>(ss, ls, s0, pm_hd, cfg);
+    const int tbl = ic < kSchedGenericIc ? ic : kSchedGenericIc;
+    const uint16_t *masks = cfg.sched + (size_t)tbl * kSchedStride;
+    const uint32_t *words = pos_tables(cfg.sched) + (size_t)tbl * kSchedStride;
+#pragma unroll 1
+    while (!ss.done) {
+        const uint32_t t = ss.t;
+        const uint32_t w = words[t];
+        if (w == 0u || (FINITE_T && (w & kPosLevel) && !((uint64_t)t + 1u < cfg.tmax))) {
+            ss.done = true; // end of the schedule, or the next weight level is over budget
+            break;
+        }
+        double pm = pm_hd;
+        uint32_t syn = s0;
+#pragma unroll 1
+        for (uint32_t r = masks[t]; r; r &= r - 1u) {
+            const int b = __builtin_ctz(r);
+            pm = pm + (double)S.d(b);
+            syn ^= (uint32_t)S.h(b);
+        }
+        const double e = exp_neg(pm, cfg.exptab);
+        ss.t = t + 1;
+        ss.pnot -= e;
+        if ((syn & 0xffu) == 0 && list_hit(ls, e, t + 1u, ss.pnot, cfg)) ss.done = true;
+    }
+}
+
 // Further guesses of the schedule, in chunks of 8 position words (four pairs,
 // 32 bytes), up to about max_steps guesses (rounded up to 16). A search starts
 // at a chunk boundary and a lane that stops mid-chunk is done, so every
@@ -718,6 +812,14 @@ __device__ __forceinline__ void search_run(SearchState &ss, LineList &ls, const 
             if (!ss.done) search_pair<FINITE_T, P>(ss, ls, S, s0, pm_hd, ic, cfg, b1.x, b1.y, t + 4u);
             if (!ss.done) search_pair<FINITE_T, P>(ss, ls, S, s0, pm_hd, ic, cfg, b1.z, b1.w, t + 6u);
         }
+    }
+    if constexpr (!P::kF32 && !P::kLegacy) {
+        // a search that ended with P_notGuess below 2^-680 (rounding takes it
+        // to 0 or below in about 2 % of lines) and met a metric from 512 on
+        // (0x40800000 is the high word of 512.0) may have needed that metric's
+        // exact weight (exp_neg_loop): it is run again with exact weights
+        if (ss.done && !chain_exact(ss.pnot) && ss.pm_hi >= 0x40800000u)
+            search_exact<FINITE_T, P>(ss, ls, S, hd, s0, pm_hd, ic, cfg);
     }
 }
 

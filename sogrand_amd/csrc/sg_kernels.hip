@@ -85,15 +85,13 @@ __global__ __launch_bounds__(kPlainThreads) void plain_kernel(CubeJob job)
     __shared__ uint16_t s_hp[17 * kPlainThreads];
     __shared__ uint8_t s_hd[kCube];
     __shared__ uint32_t s_sum[kPlainThreads / 64], s_max[kPlainThreads / 64];
-    __shared__ ExpEntry s_exp[kExpTableEntries];
-    __shared__ LogEntry s_log[kLogTableEntries];
+    __shared__ ExpEntry s_exp[kLdsTableEntries];
 
     const int t = threadIdx.x;
-    load_exp_table(s_exp, t);
-    load_log_table(s_log, t);
+    load_exp_table(s_exp, t, job.eps, job.thres);
     __syncthreads();
     const LineLdsT<Rec, kPlainThreads> S(s_delta, s_hp, t);
-    const DecodeCfg cfg{job.tmax, job.thres, job.sched, job.L, s_exp, s_log, job.eps, job.eps_log};
+    const DecodeCfg cfg{job.tmax, job.thres, job.sched, job.L, s_exp, kLogTable, job.eps, job.eps_log};
     double *le = job.le_scratch + (size_t)blockIdx.x * kCube;
 
     for (size_t cube = blockIdx.x; cube < job.n_cw; cube += gridDim.x) {
@@ -202,13 +200,11 @@ __global__ __launch_bounds__(256) void lines_kernel(const double *llr, size_t n,
     using Rec = typename P::Rec;
     __shared__ Rec s_delta[17 * 256];
     __shared__ uint16_t s_hp[17 * 256];
-    __shared__ ExpEntry s_exp[kExpTableEntries];
-    __shared__ LogEntry s_log[kLogTableEntries];
-    load_exp_table(s_exp, threadIdx.x);
-    load_log_table(s_log, threadIdx.x);
+    __shared__ ExpEntry s_exp[kLdsTableEntries];
+    load_exp_table(s_exp, threadIdx.x, cfg.eps, cfg.thres);
     __syncthreads();
     cfg.exptab = s_exp;
-    cfg.logtab = s_log;
+    cfg.logtab = kLogTable;
     const size_t line = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (line >= n) return;
     const LineLdsT<Rec, 256> S(s_delta, s_hp, (int)threadIdx.x);

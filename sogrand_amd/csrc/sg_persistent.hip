@@ -158,18 +158,16 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))
     __shared__ double s_stage_jn[kWaves][64];
     __shared__ double s_stage_a[kWaves][4][2]; // per sub-round line: |x| of ranks 0 and 7
     __shared__ uint32_t s_geom[kWaves][S][2];  // per slot: cube, axis | iteration << 8
-    __shared__ ExpEntry s_exp[kExpTableEntries];
-    __shared__ LogEntry s_log[kLogTableEntries];
+    __shared__ ExpEntry s_exp[kLdsTableEntries];
 
-    load_exp_table(s_exp, threadIdx.x);
-    load_log_table(s_log, threadIdx.x);
+    load_exp_table(s_exp, threadIdx.x, job.eps, job.thres);
     __syncthreads(); // the only workgroup barrier: waves run independently from here
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint64_t lt_mask = (1ull << lane) - 1ull;
     Rec *rec_d = &s_d[wave][0];
     uint16_t *rec_h = &s_h[wave][0];
     const LineLdsT<Rec, 64> L(rec_d, rec_h, lane);
-    const DecodeCfg cfg{job.tmax, job.thres, job.sched, job.L, s_exp, s_log, job.eps, job.eps_log};
+    const DecodeCfg cfg{job.tmax, job.thres, job.sched, job.L, s_exp, kLogTable, job.eps, job.eps_log};
     const size_t wave_id = (size_t)blockIdx.x * kWaves + wave;
     uint32_t *hd_bits = &s_hd[wave][0][0];
     uint32_t *cnt = &s_cnt[wave][0][0];

@@ -92,6 +92,38 @@ def test_cubes_match_reference_binary(decoder, oracle, backend, snr):
          f"the reference's FMA build: {frac_within(g['soft_fma'], g['soft']):.3f} within 1e-9")
 
 
+def test_lines_with_large_metrics_match_oracle(decoder, oracle):
+    # magnitudes far above JacLog's cut at 50: the hard decision's weight is
+    # exp(-0) = 1, so P_notGuess starts at 0 and goes negative (the search's
+    # exact re-run), and hits have metrics above 512 (the library's
+    # large-argument exp); plus lines mixing such magnitudes with ordinary ones
+    rng = np.random.default_rng(77)
+    n = 512
+    mag = rng.uniform(60.0, 900.0, (n, 16))
+    mag[n // 2:, :6] = rng.uniform(0.0, 4.0, (n // 2, 6))
+    x = mag * rng.choice([1.0, -1.0], (n, 16))
+    la, le, ng = decoder.bitso(x)
+    for i in range(n):
+        la2, le2, tr = oracle.siso(x[i])
+        assert ng[i] == tr.T, i
+        assert soft_close(la[i], la2), (i, soft_err(la[i], la2))
+        assert soft_close(le[i], le2), (i, soft_err(le[i], le2))
+
+
+@pytest.mark.parametrize("backend", [0, 1], ids=["plain", "persistent"])
+def test_cubes_with_large_llrs_match_oracle(decoder, oracle, backend):
+    # channel LLRs scaled far beyond the channel's range (metrics above 512,
+    # negative P_notGuess chains), whole decodes
+    sg = _sg()
+    llr = sg.gen_llr(6060, 24, ebn0_db=1.0) * np.linspace(20.0, 400.0, 24)[:, None]
+    decoder.set_params(backend=backend)
+    r = decoder.decode(llr, soft=True)
+    bits, st, soft = oracle.decode(llr, soft=True)
+    assert (r.bits == bits).all()
+    assert (r.stats == st).all()
+    assert soft_close(r.soft, soft), soft_err(r.soft, soft)
+
+
 @pytest.mark.parametrize("backend", [0, 1], ids=["plain", "persistent"])
 def test_cubes_2000_match_oracle(decoder, oracle, backend):
     # the 2,000-codeword subset at 2 dB, checked against the oracle
