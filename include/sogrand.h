@@ -40,6 +40,7 @@ extern "C" {
 #define SOGRAND_ENODEV (-5)     /* no usable gfx950 device                     */
 #define SOGRAND_EIO (-6)        /* file open/read/write failed (CLI helpers)   */
 #define SOGRAND_EBUSY (-7)      /* serving queue full / admission refused      */
+#define SOGRAND_ETOOBIG (-8)    /* more codewords than a fixed-size entry point takes */
 
 /* precision */
 #define SOGRAND_FP64 0       /* bit-exact with SOGRAND_C                              */
@@ -114,6 +115,23 @@ void sogrand_ctx_destroy(sogrand_ctx *ctx);
  * highly_optimized_sogrand_kernel, cubic_decoder1.cu:635-643). */
 int sogrand_decode_cubic(sogrand_ctx *ctx, const double *llr, size_t n_cw, uint8_t *info_bits,
                          double *soft_out, sogrand_cw_stats *stats, void *hip_stream);
+
+/* sogrand_decode_cubic without any allocation. It has no counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * llr, info_bits and the optional soft_out / stats must be device memory
+ * (SOGRAND_EINVAL otherwise); at most SOGRAND_NOALLOC_MAX_CW codewords per
+ * call (SOGRAND_ETOOBIG otherwise). It runs the plain kernel whatever the
+ * context's backend (results are the same bit for bit), with its device
+ * scratch in a static array of SOGRAND_NOALLOC_SCRATCH_BYTES that every GPU
+ * the library runs on holds until the process ends. All calls on one GPU share
+ * that array: calls that overlap on one GPU (from several contexts or threads)
+ * race. Synchronous, like sogrand_decode_cubic. */
+#define SOGRAND_NOALLOC_MAX_CW 1048576
+#define SOGRAND_NOALLOC_SCRATCH_BYTES (32u << 20)
+int sogrand_decode_cubic_noalloc(sogrand_ctx *ctx, const double *llr, size_t n_cw,
+                                 uint8_t *info_bits, double *soft_out, sogrand_cw_stats *stats,
+                                 void *hip_stream);
 
 /* Component SISO decode of n_lines independent 16-value lines on the GPU:
  * L_APP, L_E (16 f64 per line) and N_guess per line. Replaces SOGRAND_bitSO

@@ -29,6 +29,7 @@ ENONFINITE = -4
 ENODEV = -5
 EIO = -6
 EBUSY = -7
+ETOOBIG = -8
 
 FP64 = 0
 FP32_MIXED = 1
@@ -45,7 +46,8 @@ CW_STATS_DTYPE = np.dtype(
 # The C-ABI symbols include/sogrand.h declares.
 EXPORTED = [
     "sogrand_params_default", "sogrand_strerror", "sogrand_ctx_create", "sogrand_ctx_set_params",
-    "sogrand_ctx_destroy", "sogrand_decode_cubic", "sogrand_bitso_lines", "sogrand_last_kernel_ms",
+    "sogrand_ctx_destroy", "sogrand_decode_cubic", "sogrand_decode_cubic_noalloc",
+    "sogrand_bitso_lines", "sogrand_last_kernel_ms",
     "sogrand_graph_replays",
     "sogrand_sigma_for_ebn0", "sogrand_gen_info_bits", "sogrand_encode_cubes", "sogrand_gen_llr",
     "sogrand_gen_llr_device", "sogrand_schedule_table", "sogrand_server_cfg_default",
@@ -128,6 +130,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.sogrand_ctx_destroy.restype = None
     lib.sogrand_decode_cubic.argtypes = [vp, vp, C.c_size_t, vp, vp, vp, vp]
     lib.sogrand_bitso_lines.argtypes = [vp, vp, C.c_size_t, vp, vp, vp, vp]
+    lib.sogrand_decode_cubic_noalloc.argtypes = [vp, vp, C.c_size_t, vp, vp, vp, vp]
     lib.sogrand_last_kernel_ms.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_uint32)]
     lib.sogrand_graph_replays.argtypes = [vp, C.POINTER(C.c_uint32)]
     lib.sogrand_sigma_for_ebn0.argtypes = [C.c_double]
@@ -250,6 +253,13 @@ class Decoder:
                                             stream)
         _check(rc, "decode_cubic")
         return DecodeResult(bits, st, sf)
+
+    def decode_noalloc(self, llr, out_bits, out_stats=None, out_soft=None, stream=None) -> None:
+        """sogrand_decode_cubic_noalloc: device buffers only, nothing allocated."""
+        n = llr.numel() // 4096
+        _check(self._lib.sogrand_decode_cubic_noalloc(self._h, _ptr(llr), n, _ptr(out_bits),
+                                                      _ptr(out_soft), _ptr(out_stats), stream),
+               "decode_cubic_noalloc")
 
     def bitso(self, lines):
         """Component SISO on (n, 16) lines -> (L_APP, L_E, N_guess)."""

@@ -37,6 +37,7 @@ COMMON = [
 
 LIB_SOURCES = [
     "sg_kernels.hip", "sg_persistent.hip", "sg_api.cpp", "sg_schedule.cpp", "sg_serve.cpp",
+    "sg_noalloc.hip",
 ]
 
 

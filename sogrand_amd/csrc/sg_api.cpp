@@ -151,6 +151,7 @@ const char *sogrand_strerror(int code)
     case SOGRAND_ENODEV: return "no gfx950 HIP device";
     case SOGRAND_EIO: return "file I/O error";
     case SOGRAND_EBUSY: return "admission refused";
+    case SOGRAND_ETOOBIG: return "too many codewords for this entry point";
     default: return "unknown error";
     }
 }
@@ -231,6 +232,11 @@ int sogrand_ctx_create(sogrand_ctx **out, int device, const sogrand_params *para
         return SOGRAND_ENOMEM;
     }
     if (hipMemcpy(c->d_sched, buf.data(), buf.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        sogrand_ctx_destroy(c);
+        return SOGRAND_EHIP;
+    }
+    // the no-allocation entry point's scratch on this device (sg_noalloc.hip)
+    if (sg::noalloc_scratch_init(device) != hipSuccess) {
         sogrand_ctx_destroy(c);
         return SOGRAND_EHIP;
     }

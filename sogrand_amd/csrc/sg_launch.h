@@ -48,6 +48,9 @@ size_t plain_scratch_slots(size_t n_cw);
 // `slots` cubes in flight and hands lines to lanes as they free up.
 hipError_t launch_persistent(const CubeJob &job, int waves_per_block, int blocks, int slots,
                              uint32_t lane_batch, uint32_t step_len, hipStream_t st);
+// Device address of the no-allocation entry point's static scratch on
+// `device` (current), taken once per device (sg_noalloc.hip).
+hipError_t noalloc_scratch_init(int device);
 // Resident workgroups per CU of the persistent kernel (occupancy API).
 int persistent_blocks_per_cu(int slots, bool finite, int arith);
 constexpr int kPersistentWaves = 4;
