@@ -36,11 +36,11 @@ extern "C" {
 #define SOGRAND_EINVAL (-1)     /* bad argument                               */
 #define SOGRAND_ENOMEM (-2)     /* host or device allocation failed          */
 #define SOGRAND_EHIP (-3)       /* a HIP runtime call failed                   */
-#define SOGRAND_ENONFINITE (-4) /* an LLR was NaN or infinite                  */
+#define SOGRAND_ENONFINITE (-4) /* not returned: every double is decoded as the
+                                   reference decodes it (+-inf and NaN included) */
 #define SOGRAND_ENODEV (-5)     /* no usable gfx950 device                     */
 #define SOGRAND_EIO (-6)        /* file open/read/write failed (CLI helpers)   */
 #define SOGRAND_EBUSY (-7)      /* serving queue full / admission refused      */
-#define SOGRAND_ETOOBIG (-8)    /* more codewords than a fixed-size entry point takes */
 
 /* precision */
 #define SOGRAND_FP64 0       /* bit-exact with SOGRAND_C                              */
@@ -120,14 +120,14 @@ int sogrand_decode_cubic(sogrand_ctx *ctx, const double *llr, size_t n_cw, uint8
  * reference: it is for callers that cannot allocate device memory (code under
  * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
  * llr, info_bits and the optional soft_out / stats must be device memory
- * (SOGRAND_EINVAL otherwise); at most SOGRAND_NOALLOC_MAX_CW codewords per
- * call (SOGRAND_ETOOBIG otherwise). It runs the plain kernel whatever the
- * context's backend (results are the same bit for bit), with its device
- * scratch in a static array of SOGRAND_NOALLOC_SCRATCH_BYTES that every GPU
- * the library runs on holds until the process ends. All calls on one GPU share
- * that array: calls that overlap on one GPU (from several contexts or threads)
- * race. Synchronous, like sogrand_decode_cubic. */
-#define SOGRAND_NOALLOC_MAX_CW 1048576
+ * (SOGRAND_EINVAL otherwise). It runs the plain kernel whatever the context's
+ * backend (results are the same bit for bit), with at most
+ * SOGRAND_NOALLOC_SCRATCH_BYTES / 32 KiB = 1024 workgroups, each looping over
+ * codewords with one 32 KiB scratch cube; the scratch is a static array of
+ * SOGRAND_NOALLOC_SCRATCH_BYTES that every GPU the library runs on holds until
+ * the process ends, so it fits any n_cw. All calls on one GPU share that array:
+ * calls that overlap on one GPU (from several contexts or threads) race.
+ * Synchronous, like sogrand_decode_cubic. */
 #define SOGRAND_NOALLOC_SCRATCH_BYTES (32u << 20)
 int sogrand_decode_cubic_noalloc(sogrand_ctx *ctx, const double *llr, size_t n_cw,
                                  uint8_t *info_bits, double *soft_out, sogrand_cw_stats *stats,
@@ -214,7 +214,7 @@ typedef struct sogrand_server_cfg {
 
 typedef struct sogrand_completion {
     uint64_t id;     /* the id given to sogrand_server_submit                         */
-    int32_t status;  /* SOGRAND_OK, SOGRAND_ENONFINITE or SOGRAND_EHIP                 */
+    int32_t status;  /* SOGRAND_OK or SOGRAND_EHIP                                      */
     uint32_t n_cw;
     double t_submit; /* seconds on the server clock: submitted, batch launched, results */
     double t_start;  /* copied back                                                     */

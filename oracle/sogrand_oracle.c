@@ -298,7 +298,6 @@ int og_siso(const double *llr, int L, uint64_t Tmax, double thres, double *L_APP
     unsigned hd = 0;
     uint8_t s0 = 0;
     for (int i = 0; i < 16; i++) {
-        if (!isfinite(llr[i])) return -1; /* the decoders' contract: finite LLRs only */
         if (!(llr[i] > 0.0)) { /* HardDec (:758) */
             hd |= 1u << i;
             s0 ^= og_hcol[i];

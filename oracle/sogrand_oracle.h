@@ -69,7 +69,8 @@ void og_set_epsilon(double eps);
  * the search, clamps 1e-9 and 1e-30). */
 void og_set_variant(int variant);
 
-/* Component SISO decode of one line. Returns 0, or -1 on a non-finite input.
+/* Component SISO decode of one line (any doubles, +-inf and NaN as the
+ * reference takes them). Returns 0.
  * L_APP / L_E may be NULL; trace may be NULL. */
 int og_siso(const double *llr, int L, uint64_t Tmax, double thres, double *L_APP, double *L_E,
             og_line_trace *trace);
@@ -80,7 +81,7 @@ int og_cube_is_codeword(const double *L_APP);
 /* Decode n_cw cubes (4096 f64 each, index a0 + 16*a1 + 256*a2).
  * bits: 64 B per codeword, MSB-first packed info bits b = a0 + 8*a1 + 64*a2.
  * soft (optional): final L_APP, 4096 f64 per codeword. stats (optional).
- * threads <= 0 means all online CPUs. Returns 0, or -1 if any input is not finite. */
+ * threads <= 0 means all online CPUs. Returns 0. */
 int og_decode_cubes(const double *llr, size_t n_cw, int L, int Imax, uint64_t Tmax, double p_et,
                     double alpha, uint8_t *bits, double *soft, og_cw_stats *stats, int threads);
 

@@ -29,7 +29,6 @@ ENONFINITE = -4
 ENODEV = -5
 EIO = -6
 EBUSY = -7
-ETOOBIG = -8
 
 FP64 = 0
 FP32_MIXED = 1

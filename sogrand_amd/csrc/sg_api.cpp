@@ -147,11 +147,10 @@ const char *sogrand_strerror(int code)
     case SOGRAND_EINVAL: return "invalid argument";
     case SOGRAND_ENOMEM: return "out of memory";
     case SOGRAND_EHIP: return "HIP runtime error";
-    case SOGRAND_ENONFINITE: return "LLR input is NaN or infinite";
+    case SOGRAND_ENONFINITE: return "non-finite LLR (no longer returned)";
     case SOGRAND_ENODEV: return "no gfx950 HIP device";
     case SOGRAND_EIO: return "file I/O error";
     case SOGRAND_EBUSY: return "admission refused";
-    case SOGRAND_ETOOBIG: return "too many codewords for this entry point";
     default: return "unknown error";
     }
 }
@@ -266,7 +265,6 @@ void sogrand_ctx_destroy(sogrand_ctx *ctx)
     ctx->info_stage.release();
     ctx->stats_stage.release();
     ctx->queue.release();
-    ctx->status.release();
     ctx->nguess_stage.release();
     for (HostStage &h : ctx->hs) {
         if (h.st) (void)hipStreamSynchronize(h.st);
@@ -276,7 +274,6 @@ void sogrand_ctx_destroy(sogrand_ctx *ctx)
         h.info.release();
         h.stats.release();
         h.queue.release();
-        h.status.release();
         if (h.ev0) (void)hipEventDestroy(h.ev0);
         if (h.ev1) (void)hipEventDestroy(h.ev1);
         if (h.st) (void)hipStreamDestroy(h.st);
@@ -296,13 +293,11 @@ static int decode_device_chunk(sogrand_ctx *ctx, const double *d_llr, size_t n, 
                                bool use_graph = false)
 {
     const sogrand_params &p = ctx->p;
-    if (ctx->status.reserve(n) != hipSuccess) return SOGRAND_ENOMEM;
     sg::CubeJob job{};
     job.llr = d_llr;
     job.info = d_info;
     job.soft = d_soft;
     job.stats = d_stats;
-    job.status = ctx->status.p;
     job.n_cw = n;
     sg::fill_job(job, ctx, p);
     const sg::LaunchPlan lp = sg::plan_launch(ctx, p, n);
@@ -322,7 +317,6 @@ static int decode_device_chunk(sogrand_ctx *ctx, const double *d_llr, size_t n, 
         k.stats = d_stats;
         k.le = job.le_scratch;
         k.queue = job.queue;
-        k.status = job.status;
         k.n = n;
         k.st = st;
         std::memcpy(&k.p, &p, sizeof p);
@@ -359,12 +353,6 @@ static int decode_device_chunk(sogrand_ctx *ctx, const double *d_llr, size_t n, 
     float t = 0.f;
     (void)hipEventElapsedTime(&t, ctx->ev0, ctx->ev1);
     *ms += t;
-    std::vector<int32_t> status(n);
-    if (hipMemcpy(status.data(), ctx->status.p, n * sizeof(int32_t), hipMemcpyDeviceToHost) !=
-        hipSuccess)
-        return SOGRAND_EHIP;
-    for (size_t i = 0; i < n; ++i)
-        if (status[i]) return SOGRAND_ENONFINITE;
     return SOGRAND_OK;
 }
 
@@ -377,12 +365,6 @@ static int finish_stage(sogrand_ctx *ctx, HostStage &h, uint8_t *info_bits, doub
     h.busy = false;
     if (hipStreamSynchronize(h.st) != hipSuccess) return SOGRAND_EHIP;
     const size_t off = h.off, n = h.n;
-    std::vector<int32_t> status(n);
-    if (hipMemcpy(status.data(), h.status.p, n * sizeof(int32_t), hipMemcpyDeviceToHost) !=
-        hipSuccess)
-        return SOGRAND_EHIP;
-    for (size_t i = 0; i < n; ++i)
-        if (status[i]) return SOGRAND_ENONFINITE;
     if (!dev_info && hipMemcpy(info_bits + off * sg::kInfoBytes, h.info.p, n * sg::kInfoBytes,
                                hipMemcpyDeviceToHost) != hipSuccess)
         return SOGRAND_EHIP;
@@ -456,13 +438,11 @@ static int decode_host_pipelined(sogrand_ctx *ctx, const double *llr, size_t n_c
                 d_stats = h.stats.p;
             }
         }
-        if (h.status.reserve(n) != hipSuccess) { rc = SOGRAND_ENOMEM; break; }
         sg::CubeJob job{};
         job.llr = d_llr;
         job.info = d_info;
         job.soft = d_soft;
         job.stats = d_stats;
-        job.status = h.status.p;
         job.n_cw = n;
         sg::fill_job(job, ctx, p);
         const sg::LaunchPlan lp = sg::plan_launch(ctx, p, n);
@@ -594,15 +574,13 @@ int sogrand_bitso_lines(sogrand_ctx *ctx, const double *llr, size_t n_lines, dou
     // stage everything through one device buffer: llr | la | le
     if (ctx->llr_stage.reserve(n_lines * 16 * 3) != hipSuccess) return SOGRAND_ENOMEM;
     if (ctx->nguess_stage.reserve(n_lines) != hipSuccess) return SOGRAND_ENOMEM;
-    if (ctx->status.reserve(n_lines) != hipSuccess) return SOGRAND_ENOMEM;
     double *d_in = ctx->llr_stage.p, *d_la = d_in + n_lines * 16, *d_le = d_la + n_lines * 16;
     if (hipMemcpyAsync(d_in, llr, n_lines * 16 * sizeof(double), hipMemcpyDefault, st) !=
         hipSuccess)
         return SOGRAND_EHIP;
     double eps, eps_log;
     sg::eps_of(ctx->p, eps, eps_log);
-    if (sg::launch_lines(d_in, n_lines, d_la, d_le, ctx->nguess_stage.p, ctx->status.p,
-                         ctx->d_sched, effective_tmax(ctx->p.Tmax), 1.0 - ctx->p.p_ET, ctx->p.L,
+    if (sg::launch_lines(d_in, n_lines, d_la, d_le, ctx->nguess_stage.p, ctx->d_sched, effective_tmax(ctx->p.Tmax), 1.0 - ctx->p.p_ET, ctx->p.L,
                          eps, eps_log, sg::arith_of(ctx->p), st) != hipSuccess)
         return SOGRAND_EHIP;
     if (hipStreamSynchronize(st) != hipSuccess) return SOGRAND_EHIP;
@@ -615,12 +593,6 @@ int sogrand_bitso_lines(sogrand_ctx *ctx, const double *llr, size_t n_lines, dou
     if (N_guess && hipMemcpy(N_guess, ctx->nguess_stage.p, n_lines * sizeof(uint32_t),
                              hipMemcpyDefault) != hipSuccess)
         return SOGRAND_EHIP;
-    std::vector<int32_t> status(n_lines);
-    if (hipMemcpy(status.data(), ctx->status.p, n_lines * sizeof(int32_t),
-                  hipMemcpyDeviceToHost) != hipSuccess)
-        return SOGRAND_EHIP;
-    for (size_t i = 0; i < n_lines; ++i)
-        if (status[i]) return SOGRAND_ENONFINITE;
     return SOGRAND_OK;
 }
 

@@ -48,14 +48,14 @@ inline uint64_t effective_tmax(uint64_t t) { return t == 0 ? 0x7fffffffull : t; 
 
 // A captured decode launch sequence and what it was captured for.
 struct GraphKey {
-    const void *llr, *info, *soft, *stats, *le, *queue, *status;
+    const void *llr, *info, *soft, *stats, *le, *queue;
     size_t n;
     hipStream_t st;
     sogrand_params p;
 };
 
 // One of the two alternating stages of a decode of host buffers: its own
-// stream, staging buffers, L_E scratch, work counter, status and timing events,
+// stream, staging buffers, L_E scratch, work counter and timing events,
 // so that one chunk's input copy and kernel overlap the previous chunk's kernel.
 struct HostStage {
     hipStream_t st = nullptr;
@@ -63,7 +63,6 @@ struct HostStage {
     sg::DevBuf<double> llr, soft, le;
     sg::DevBuf<uint8_t> info;
     sg::DevBuf<uint32_t> stats, queue;
-    sg::DevBuf<int32_t> status;
     size_t off = 0, n = 0; // the chunk in flight
     bool busy = false;
 };
@@ -76,7 +75,6 @@ struct sogrand_ctx {
     sg::DevBuf<double> le_scratch, llr_stage, soft_stage, sigma_stage;
     sg::DevBuf<uint8_t> info_stage;
     sg::DevBuf<uint32_t> stats_stage, queue;
-    sg::DevBuf<int32_t> status;
     sg::DevBuf<uint32_t> nguess_stage;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;

@@ -311,32 +311,30 @@ template <class LDS> __device__ __forceinline__ void place_by_rank(const LDS &S,
 //      of the permutation in place;
 //   4. with ties, the reference's quicksort decides the order among equal
 //      magnitudes, so it is replayed on the records (quicksort_lds).
-// Returns false if an input is not finite. legacy: cubic_decoder.c's JacLog
-// cut and IC guard.
+// Every double is taken as the reference takes it: an infinite magnitude
+// gives an infinite increment, and a NaN flows through the sort (as a tie), the
+// intercept and the metrics. legacy: cubic_decoder.c's JacLog cut and IC guard.
 template <class Src, class LDS>
-__device__ __forceinline__ bool line_setup(const Src &src, const LDS &S, uint32_t &hd,
+__device__ __forceinline__ void line_setup(const Src &src, const LDS &S, uint32_t &hd,
                                            uint32_t &s0, double &pm_hd, int32_t &ic,
                                            bool legacy, const DecodeCfg &cfg)
 {
     const double jlim = legacy ? 30.0 : 50.0;
-    bool finite = true;
     hd = 0;
     s0 = 0;
 #pragma unroll 1
     for (int i = 0; i < 16; ++i) {
         const double x = src(i);
-        finite = finite && __builtin_isfinite(x);
         if (!(x > 0.0)) { // HardDec (:758)
             hd |= 1u << i;
             s0 ^= hcol_of((unsigned)i);
         }
         S.d(i) = fabs(x);
     }
-    if (!finite) return false;
 
     // rank = the number of smaller magnitudes: a permutation exactly when the
-    // magnitudes are distinct, so a repeated rank (a missing bit in seen)
-    // reveals a tie, which the quicksort replay below orders
+    // magnitudes are distinct and not NaN, so a repeated rank (a missing bit in
+    // seen) reveals a tie or a NaN, which the quicksort replay below orders
     uint64_t rank = 0; // 4 bits per natural position
     uint32_t seen = 0;
     double a_first = 0.0, a_eighth = 0.0;
@@ -371,39 +369,39 @@ __device__ __forceinline__ bool line_setup(const Src &src, const LDS &S, uint32_
     }
     S.d(-1) = 0.0; // null record read by empty position-word fields
     S.h(-1) = 0;
-    return true;
 }
 
 // Set-up of the single-precision variant: reliabilities ranked as floats (equal
 // floats ordered by position), IC from the two float order statistics, PM_HD
 // in double in natural order, increments computed in double and rounded to
-// float. Returns false if an input is not finite.
+// float.
 template <class Src, class LDS>
-__device__ __forceinline__ bool line_setup_f32(const Src &src, const LDS &S,
+__device__ __forceinline__ void line_setup_f32(const Src &src, const LDS &S,
                                                uint32_t &hd, uint32_t &s0, double &pm_hd,
                                                int32_t &ic, const DecodeCfg &cfg)
 {
-    bool finite = true;
     hd = 0;
 #pragma unroll 1
     for (int i = 0; i < 16; ++i) {
         const double x = src(i);
-        finite = finite && __builtin_isfinite(x);
         if (!(x > 0.0)) hd |= 1u << i; // HardDec (:758)
         S.d(i) = (float)fabs(x);
     }
     s0 = syndrome16(hd);
-    if (!finite) return false;
+    // ranks over the keys' bit patterns, ties by position: the keys are >= 0,
+    // so this is their order (a NaN key sorts last), and always a permutation,
+    // which place_by_rank's cycle walk needs
     uint64_t rank = 0;
     float a_first = 0.0f, a_eighth = 0.0f;
 #pragma unroll 1
     for (int i = 0; i < 16; ++i) {
         const float ai = S.d(i);
+        const uint32_t bi = __builtin_bit_cast(uint32_t, ai);
         uint32_t r = 0;
 #pragma unroll 4
         for (int j = 0; j < 16; ++j) {
-            const float aj = S.d(j);
-            r += (aj < ai || (aj == ai && j < i)) ? 1u : 0u;
+            const uint32_t bj = __builtin_bit_cast(uint32_t, (float)S.d(j));
+            r += (bj < bi || (bj == bi && j < i)) ? 1u : 0u;
         }
         rank |= (uint64_t)r << (4 * i);
         if (r == 0) a_first = ai;
@@ -421,19 +419,18 @@ __device__ __forceinline__ bool line_setup_f32(const Src &src, const LDS &S,
     place_by_rank(S, rank);
     S.d(-1) = 0.0f; // null record
     S.h(-1) = 0;
-    return true;
 }
 
 // Set-up in the arithmetic of P.
 template <class P, class Src, class LDS>
-__device__ __forceinline__ bool line_setup_p(const Src &src, const LDS &S,
+__device__ __forceinline__ void line_setup_p(const Src &src, const LDS &S,
                                              uint32_t &hd, uint32_t &s0, double &pm_hd,
                                              int32_t &ic, const DecodeCfg &cfg)
 {
     if constexpr (P::kF32)
-        return line_setup_f32(src, S, hd, s0, pm_hd, ic, cfg);
+        line_setup_f32(src, S, hd, s0, pm_hd, ic, cfg);
     else
-        return line_setup(src, S, hd, s0, pm_hd, ic, P::kLegacy, cfg);
+        line_setup(src, S, hd, s0, pm_hd, ic, P::kLegacy, cfg);
 }
 
 // Record a list entry and apply the stopping rule (:628-641, :674-686, getLConf :767-774).
@@ -817,8 +814,11 @@ __device__ __forceinline__ void search_run(SearchState &ss, LineList &ls, const 
         // a search that ended with P_notGuess below 2^-680 (rounding takes it
         // to 0 or below in about 2 % of lines) and met a metric from 512 on
         // (0x40800000 is the high word of 512.0) may have needed that metric's
-        // exact weight (exp_neg_loop): it is run again with exact weights
-        if (ss.done && !chain_exact(ss.pnot) && ss.pm_hi >= 0x40800000u)
+        // exact weight (exp_neg_loop), and one that met a NaN metric (high word
+        // above +inf's 0x7ff00000; exp(-NaN) is NaN, the stand-in is not) needs
+        // it: such searches are run again with exact weights
+        if (ss.done && ((!chain_exact(ss.pnot) && ss.pm_hi >= 0x40800000u) ||
+                        ss.pm_hi > 0x7ff00000u))
             search_exact<FINITE_T, P>(ss, ls, S, hd, s0, pm_hd, ic, cfg);
     }
 }

@@ -99,7 +99,6 @@ __global__ __launch_bounds__(kPlainThreads) void plain_kernel(CubeJob job)
         double *soft = job.soft ? job.soft + cube * kCube : nullptr;
         uint32_t half = 0, conv = 0;
         uint64_t ng = 0, ngp = 0;
-        bool bad = false;
         for (uint32_t it = 1; it <= job.Imax && !conv; ++it) {
             for (int ax = 0; ax < 3; ++ax) {
                 ++half; // n_iter += 0.5 before each phase (:145, :184, :223)
@@ -109,14 +108,8 @@ __global__ __launch_bounds__(kPlainThreads) void plain_kernel(CubeJob job)
                 double pm_hd;
                 int32_t ic;
                 LineList ls;
-                if (line_setup_p<P>(src, S, hd, s0, pm_hd, ic, cfg)) {
-                    line_search<FINITE_T, P>(S, hd, s0, pm_hd, ic, cfg, ls);
-                } else {
-                    bad = true;
-                    ls.T = 0;
-                    ls.nl = 0;
-                    ls.pnl = 0.0;
-                }
+                line_setup_p<P>(src, S, hd, s0, pm_hd, ic, cfg);
+                line_search<FINITE_T, P>(S, hd, s0, pm_hd, ic, cfg, ls);
                 const double pnotin = fmax(ls.pnl, cfg.eps);
                 ListEntries ent;
                 list_entries<P>(ls, S, hd, pm_hd, ic, cfg, ent);
@@ -187,15 +180,13 @@ __global__ __launch_bounds__(kPlainThreads) void plain_kernel(CubeJob job)
                 st->NG_p = ngp;
             }
         }
-        const int any = __syncthreads_or(bad);
-        if (t == 0) job.status[cube] = any ? 1 : 0;
+        __syncthreads(); // s_hd is read above before the next cube overwrites it
     }
 }
 
 template <bool FINITE_T, class P>
 __global__ __launch_bounds__(256) void lines_kernel(const double *llr, size_t n, double *la,
-                                                    double *le, uint32_t *nguess,
-                                                    int32_t *status, DecodeCfg cfg)
+                                                    double *le, uint32_t *nguess, DecodeCfg cfg)
 {
     using Rec = typename P::Rec;
     __shared__ Rec s_delta[17 * 256];
@@ -214,14 +205,8 @@ __global__ __launch_bounds__(256) void lines_kernel(const double *llr, size_t n,
     double pm_hd;
     int32_t ic;
     LineList ls;
-    const bool ok = line_setup_p<P>(src, S, hd, s0, pm_hd, ic, cfg);
-    if (ok) {
-        line_search<FINITE_T, P>(S, hd, s0, pm_hd, ic, cfg, ls);
-    } else {
-        ls.T = 0;
-        ls.nl = 0;
-        ls.pnl = 0.0;
-    }
+    line_setup_p<P>(src, S, hd, s0, pm_hd, ic, cfg);
+    line_search<FINITE_T, P>(S, hd, s0, pm_hd, ic, cfg, ls);
     const double pnotin = fmax(ls.pnl, cfg.eps);
     ListEntries ent;
     list_entries<P>(ls, S, hd, pm_hd, ic, cfg, ent);
@@ -240,7 +225,6 @@ __global__ __launch_bounds__(256) void lines_kernel(const double *llr, size_t n,
         if (le) le[line * 16 + i] = e;
     }
     if (nguess) nguess[line] = ls.T;
-    status[line] = ok ? 0 : 1;
 }
 
 // One 256-thread workgroup per codeword: hash the info bits, encode the cube
@@ -333,7 +317,7 @@ hipError_t launch_plain(const CubeJob &job, int grid, hipStream_t st)
 }
 
 hipError_t launch_lines(const double *llr, size_t n_lines, double *la, double *le,
-                        uint32_t *nguess, int32_t *status, const uint16_t *sched, uint64_t tmax,
+                        uint32_t *nguess, const uint16_t *sched, uint64_t tmax,
                         double thres, uint32_t L, double eps, double eps_log, int arith,
                         hipStream_t st)
 {
@@ -342,7 +326,7 @@ hipError_t launch_lines(const double *llr, size_t n_lines, double *la, double *l
     const dim3 g((unsigned)((n_lines + 255) / 256)), b(256);
     const bool finite = tmax_binds(tmax);
 #define SG_LINES(FT, PP)                                                                      \
-    hipLaunchKernelGGL((lines_kernel<FT, PP>), g, b, 0, st, llr, n_lines, la, le, nguess, status, cfg)
+    hipLaunchKernelGGL((lines_kernel<FT, PP>), g, b, 0, st, llr, n_lines, la, le, nguess, cfg)
     if (arith == kArithF32) {
         if (finite) SG_LINES(true, PrecF32); else SG_LINES(false, PrecF32);
     } else if (arith == kArithF64Legacy) {

@@ -18,7 +18,6 @@ struct CubeJob {
     uint8_t *info;            // n_cw x 64 (device)
     double *soft;             // n_cw x 4096 or null (device)
     uint32_t *stats;          // n_cw x 6 u32 {half, conv, NG lo, NG hi, NGp lo, NGp hi} or null
-    int32_t *status;          // n_cw: 0 ok, 1 non-finite input (device)
     double *le_scratch;       // 4096-f64 cubes: one per plain workgroup, two per
                               // persistent wave slot (kPersistentSlotScratch)
     uint32_t *queue;          // persistent: work counter (device), zeroed by the launcher
@@ -60,7 +59,7 @@ constexpr int kPersistentSlotScratch = 2;
 
 // Line-level SISO (SOGRAND_bitSO on many lines).
 hipError_t launch_lines(const double *llr, size_t n_lines, double *la, double *le,
-                        uint32_t *nguess, int32_t *status, const uint16_t *sched, uint64_t tmax,
+                        uint32_t *nguess, const uint16_t *sched, uint64_t tmax,
                         double thres, uint32_t L, double eps, double eps_log, int arith,
                         hipStream_t st);
 

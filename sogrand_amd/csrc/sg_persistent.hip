@@ -154,7 +154,7 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))
     __shared__ Rec s_d[kWaves][17 * 64];
     __shared__ uint16_t s_h[kWaves][17 * 64];
     __shared__ uint32_t s_hd[kWaves][S][128];
-    __shared__ uint32_t s_cnt[kWaves][S][4]; // lines done, phase guess sum, phase max, bad
+    __shared__ uint32_t s_cnt[kWaves][S][3]; // lines done, phase guess sum, phase max
     __shared__ double s_stage_jn[kWaves][64];
     __shared__ double s_stage_a[kWaves][4][2]; // per sub-round line: |x| of ranks 0 and 7
     __shared__ uint32_t s_geom[kWaves][S][2];  // per slot: cube, axis | iteration << 8
@@ -213,13 +213,13 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))
 #pragma clang loop unroll(full)
         for (int s = 0; s < S; ++s) {
             if (!sl.active[s] || sl.nxt_line[s] < 256) continue;
-            const uint32_t done_lines = __builtin_amdgcn_readfirstlane(cnt[4 * s + 0]);
+            const uint32_t done_lines = __builtin_amdgcn_readfirstlane(cnt[3 * s + 0]);
             if (done_lines < 256) continue;
             // all lines of the phase are in: make their L_E stores visible to
             // the next phase's loads
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-            sl.ng[s] += __builtin_amdgcn_readfirstlane(cnt[4 * s + 1]);
-            sl.ngp[s] += __builtin_amdgcn_readfirstlane(cnt[4 * s + 2]);
+            sl.ng[s] += __builtin_amdgcn_readfirstlane(cnt[3 * s + 1]);
+            sl.ngp[s] += __builtin_amdgcn_readfirstlane(cnt[3 * s + 2]);
             // early termination: all 768 lines of the hard decision are
             // component codewords (equivalent to the re-encode test :317-391)
             bool ok = true;
@@ -250,7 +250,6 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))
                         o->NG = sl.ng[s];
                         o->NG_p = sl.ngp[s];
                     }
-                    job.status[sl.cube[s]] = cnt[4 * s + 3] ? 1 : 0;
                 }
                 sl.active[s] = false;
             } else {
@@ -265,7 +264,7 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))
                 sl.nxt_line[s] = 0;
                 hd_bits[128 * s + lane] = 0;
                 hd_bits[128 * s + 64 + lane] = 0;
-                if (lane < 3) cnt[4 * s + lane] = 0;
+                if (lane < 3) cnt[3 * s + lane] = 0;
             }
         }
         // ---- 2. refill empty slots from the device queue ----
@@ -292,7 +291,7 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))
             }
             hd_bits[128 * s + lane] = 0;
             hd_bits[128 * s + 64 + lane] = 0;
-            if (lane < 4) cnt[4 * s + lane] = 0;
+            if (lane < 3) cnt[3 * s + lane] = 0;
         }
         bool any_active = false;
         uint32_t avail = 0;
@@ -370,9 +369,9 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))
                     if (!(la > 0.0)) atomicOr(&hd_bits[128 * o_slot + (p >> 5)], 1u << (p & 31));
                 }
                 if (lane == o[0] || lane == o[1] || lane == o[2] || lane == o[3]) {
-                    atomicAdd(&cnt[4 * my_slot + 1], ss.t + 1u);
-                    atomicMax(&cnt[4 * my_slot + 2], ss.t + 1u);
-                    atomicAdd(&cnt[4 * my_slot + 0], 1u);
+                    atomicAdd(&cnt[3 * my_slot + 1], ss.t + 1u);
+                    atomicMax(&cnt[3 * my_slot + 2], ss.t + 1u);
+                    atomicAdd(&cnt[3 * my_slot + 0], 1u);
                     state = kIdle;
                 }
             }
@@ -404,7 +403,7 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))
                 const int o_slot = __shfl(my_slot, src, 64);
                 const int o_line = __shfl(my_line, src, 64);
                 const LineLdsT<Rec, 64> R(rec_d, rec_h, src); // the owner's records
-                bool fin = true, tie = false, neg = false;
+                bool tie = false, neg = false;
                 double ai = 0.0, delta = 0.0;
                 Rec key = 0;
                 uint32_t rank = 0;
@@ -419,7 +418,6 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))
                     const double tt = job.alpha * e; // L_A = alpha L_E, then + L_channel (:147-155)
                     const double x = lch[p] + tt;
                     le[kCube + 16 * o_line + i] = x; // read back by the soft output
-                    fin = __builtin_isfinite(x);
                     neg = !(x > 0.0); // HardDec (:758)
                     ai = fabs(x);
                     key = (Rec)ai;
@@ -429,14 +427,16 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))
                     // rank by counting. In double, the number of smaller
                     // magnitudes, a permutation of 0..15 over the line's lanes
                     // exactly when its magnitudes are distinct; a line with
-                    // equal magnitudes (a tie) goes to the owner, which replays
-                    // the reference's quicksort. In the single-precision variant
-                    // equal keys are ordered by position.
+                    // equal magnitudes (a tie) or a NaN goes to the owner, which
+                    // replays the reference's quicksort. In the single-precision
+                    // variant the keys' bit patterns are ranked, equal keys by
+                    // position: a permutation whatever the keys (NaN last).
                     if constexpr (P::kF32) {
+                        const uint32_t bk = __builtin_bit_cast(uint32_t, (float)key);
 #pragma unroll 4
                         for (int j = 0; j < 16; ++j) {
-                            const Rec aj = R.d(j);
-                            rank += (aj < key || (aj == key && j < i)) ? 1u : 0u;
+                            const uint32_t bj = __builtin_bit_cast(uint32_t, (float)R.d(j));
+                            rank += (bj < bk || (bj == bk && j < i)) ? 1u : 0u;
                         }
                     } else {
 #pragma unroll 4
@@ -452,9 +452,8 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))
                 }
                 // a repeated rank leaves one of the line's 16 rank bits unset
                 if constexpr (!P::kF32) tie = or16(owner >= 0 ? 1u << rank : 0u) != 0xffffu;
-                const uint64_t special = __ballot(owner >= 0 && (tie || !fin));
+                const uint64_t special = __ballot(owner >= 0 && tie);
                 const uint64_t negm = __ballot(owner >= 0 && neg);
-                const uint64_t badm = __ballot(owner >= 0 && !fin);
                 // syndrome of the hard decision: XOR of the H columns of the
                 // line's negative positions over its 16 lanes
                 const uint32_t syn16 = xor16(neg ? (uint32_t)hcol_of((unsigned)i) : 0u);
@@ -463,10 +462,9 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))
                               : lane == o[3] ? 3 : -1;
                 // the owner of line q takes the syndrome from the line's lanes
                 const uint32_t syn_owner = (uint32_t)__shfl((int)syn16, 16 * (myq >= 0 ? myq : 0), 64);
-                bool line_special = false, line_bad = false;
+                bool line_special = false;
                 if (myq >= 0) {
                     line_special = ((special >> (16 * myq)) & 0xffffull) != 0;
-                    line_bad = ((badm >> (16 * myq)) & 0xffffull) != 0;
                     hd = (uint32_t)(negm >> (16 * myq)) & 0xffffu;
                     s0 = syn_owner;
                     pm_hd = 0.0;
@@ -483,27 +481,16 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))
                 if (myq >= 0) {
                     L.d(-1) = (Rec)0; // null record read by empty position-word fields
                     L.h(-1) = 0;
-                    if (line_bad) {
-                        atomicOr(&cnt[4 * my_slot + 3], 1u);
-                        ls.nl = 0;
-                        ls.pnl = 0.0;
-                        ls.ppos = 0.0;
-                        ls.code = 0;
-                        ss.t = ~0u; // not decoded: t + 1 = 0 guesses
-                        ss.done = true;
-                        state = kDone;
-                    } else {
-                        if constexpr (!P::kF32) {
-                            // records hold |x| in natural order
-                            if (line_special) {
-                                double a_first, a_eighth;
-                                pm_hd = setup_with_ties(L, P::kJacLim, cfg, a_first, a_eighth);
-                                ic = intercept(a_first, a_eighth, P::kLegacy);
-                            }
+                    if constexpr (!P::kF32) {
+                        // records hold |x| in natural order
+                        if (line_special) {
+                            double a_first, a_eighth;
+                            pm_hd = setup_with_ties(L, P::kJacLim, cfg, a_first, a_eighth);
+                            ic = intercept(a_first, a_eighth, P::kLegacy);
                         }
-                        search_init<P>(ss, ls, hd, s0, pm_hd, ic, cfg);
-                        state = ss.done ? kDone : kSearch;
                     }
+                    search_init<P>(ss, ls, hd, s0, pm_hd, ic, cfg);
+                    state = ss.done ? kDone : kSearch;
                 }
             }
         }

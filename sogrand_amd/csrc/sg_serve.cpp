@@ -2,7 +2,7 @@
 //
 // A worker thread owns `max_inflight` batch slots. Each slot has its own
 // non-blocking HIP stream and a fixed carve of the device-memory budget (LLRs,
-// L_E scratch, bits, statistics, status of up to `max_batch_cw` codewords), so
+// L_E scratch, bits and statistics of up to `max_batch_cw` codewords), so
 // admission never allocates. Requests wait in a FIFO; whenever a slot is free,
 // every waiting request that fits is gathered into one batch (input copies on
 // the slot's stream), decoded by one launch of the configured kernel, and the
@@ -54,14 +54,14 @@ struct Slot {
     double *llr = nullptr, *le = nullptr, *soft = nullptr;
     uint8_t *info = nullptr;
     uint32_t *stats = nullptr, *queue = nullptr;
-    int32_t *status = nullptr;
     void *base = nullptr; // one device allocation per slot
-    // pinned host staging: results (info | stats | status) always, soft output
+    // pinned host staging: results (info | stats) always, soft output
     // when reserved, host-resident inputs on first use
     uint8_t *h_out = nullptr;
     double *h_soft = nullptr;
     double *h_llr = nullptr;
     bool busy = false;
+    bool launch_ok = true; // every call of the last launch() succeeded
     double t_start = 0.0;
     std::vector<Request> reqs;
 };
@@ -148,7 +148,6 @@ void sogrand_server::launch(Slot &s)
     job.info = s.info;
     job.soft = want_soft ? s.soft : nullptr;
     job.stats = s.stats;
-    job.status = s.status;
     job.le_scratch = s.le;
     job.queue = s.queue;
     job.n_cw = n;
@@ -166,9 +165,6 @@ void sogrand_server::launch(Slot &s)
     // results: device-resident outputs directly, host outputs via pinned staging
     uint8_t *h_info = s.h_out;
     uint32_t *h_stats = (uint32_t *)(s.h_out + batch_cw * sg::kInfoBytes);
-    int32_t *h_status = (int32_t *)(h_stats + batch_cw * 6);
-    ok = ok && hipMemcpyAsync(h_status, s.status, n * sizeof(int32_t), hipMemcpyDeviceToHost,
-                              s.st) == hipSuccess;
     off = 0;
     for (const Request &r : s.reqs) {
         const bool dev_info = r.info_dev;
@@ -193,6 +189,9 @@ void sogrand_server::launch(Slot &s)
         off += r.n;
     }
     ok = ok && hipEventRecord(s.done, s.st) == hipSuccess;
+    // a launch that failed part-way is finished as failed by the worker (its
+    // event may not have been recorded)
+    s.launch_ok = ok;
     if (!ok) {
         std::lock_guard<std::mutex> g(mu);
         fatal = true;
@@ -205,7 +204,6 @@ void sogrand_server::finish(Slot &s, bool event_ok)
 {
     const uint8_t *h_info = s.h_out;
     const uint32_t *h_stats = (const uint32_t *)(s.h_out + batch_cw * sg::kInfoBytes);
-    const int32_t *h_status = (const int32_t *)(h_stats + batch_cw * 6);
     size_t off = 0;
     std::vector<sogrand_completion> out;
     out.reserve(s.reqs.size());
@@ -219,9 +217,7 @@ void sogrand_server::finish(Slot &s, bool event_ok)
             if (r.soft && !r.soft_dev)
                 std::memcpy(r.soft, s.h_soft + off * sg::kCube, r.n * sg::kCube * sizeof(double));
         }
-        int32_t st = event_ok ? SOGRAND_OK : SOGRAND_EHIP;
-        for (size_t i = 0; event_ok && i < r.n; ++i)
-            if (h_status[off + i]) st = SOGRAND_ENONFINITE;
+        const int32_t st = event_ok ? SOGRAND_OK : SOGRAND_EHIP;
         out.push_back(sogrand_completion{r.id, st, (uint32_t)r.n, r.t_submit, s.t_start, t});
         off += r.n;
     }
@@ -243,6 +239,14 @@ void sogrand_server::run()
         bool progress = false;
         for (Slot &s : slots) {
             if (!s.busy) continue;
+            if (!s.launch_ok) {
+                // nothing more of it will run usefully: wait for what was
+                // enqueued, then report its requests as failed
+                (void)hipStreamSynchronize(s.st);
+                finish(s, false);
+                progress = true;
+                continue;
+            }
             const hipError_t q = hipEventQuery(s.done);
             if (q == hipErrorNotReady) continue;
             finish(s, q == hipSuccess);
@@ -313,10 +317,10 @@ int sogrand_server_create(sogrand_server **out, sogrand_ctx *ctx, const sogrand_
     s->cfg = c;
     // per-codeword bytes of a slot: LLRs, L_E scratch (up to two cubes per
     // codeword for the persistent kernel), optional soft output, bits,
-    // statistics, status
+    // statistics
     const size_t per_cw = (1 + sg::kPersistentSlotScratch) * sg::kCube * sizeof(double) +
                           (c.with_soft ? sg::kCube * sizeof(double) : 0) + sg::kInfoBytes +
-                          sizeof(sogrand_cw_stats) + sizeof(int32_t);
+                          sizeof(sogrand_cw_stats);
     size_t b = c.hbm_budget / c.max_inflight / per_cw;
     if (b > 65536) b = 65536;
     if (c.max_batch_cw && b > c.max_batch_cw) b = c.max_batch_cw;
@@ -336,8 +340,7 @@ int sogrand_server_create(sogrand_server **out, sogrand_ctx *ctx, const sogrand_
         const size_t le_n = sg::max_le_slots(b) * sg::kCube;
         const size_t bytes_d = (b * sg::kCube + le_n + (c.with_soft ? b * sg::kCube : 0)) *
                                sizeof(double);
-        const size_t bytes = bytes_d + b * 6 * sizeof(uint32_t) + b * sizeof(int32_t) + 256 +
-                             b * sg::kInfoBytes;
+        const size_t bytes = bytes_d + b * 6 * sizeof(uint32_t) + 256 + b * sg::kInfoBytes;
         if (hipMalloc(&sl.base, bytes) != hipSuccess) {
             sl.base = nullptr;
             sogrand_server_destroy(s);
@@ -350,12 +353,10 @@ int sogrand_server_create(sogrand_server **out, sogrand_ctx *ctx, const sogrand_
         q += bytes_d;
         sl.stats = (uint32_t *)q;
         q += b * 6 * sizeof(uint32_t);
-        sl.status = (int32_t *)q;
-        q += b * sizeof(int32_t);
         sl.queue = (uint32_t *)q;
         q += 256;
         sl.info = (uint8_t *)q;
-        const size_t out_bytes = b * (sg::kInfoBytes + 6 * sizeof(uint32_t) + sizeof(int32_t));
+        const size_t out_bytes = b * (sg::kInfoBytes + 6 * sizeof(uint32_t));
         if (hipHostMalloc((void **)&sl.h_out, out_bytes, hipHostMallocDefault) != hipSuccess ||
             (c.with_soft && hipHostMalloc((void **)&sl.h_soft, b * sg::kCube * sizeof(double),
                                           hipHostMallocDefault) != hipSuccess)) {

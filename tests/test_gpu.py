@@ -256,8 +256,7 @@ def test_full_size_properties(decoder):
 def test_host_buffers_in_overlapped_chunks(decoder):
     # more than 65,536 codewords from host memory: three chunks alternate
     # between the two staging streams (one stage is reused), with soft outputs
-    # and statistics; everything equals a single device-resident decode, and a
-    # non-finite value in the last chunk is still reported
+    # and statistics; everything equals a single device-resident decode
     sg = _sg()
     from sogrand_amd import hip
 
@@ -276,22 +275,46 @@ def test_host_buffers_in_overlapped_chunks(decoder):
     assert (d_soft.to_host() == h.soft).all()
     d_llr.free()
     d_soft.free()
-    llr[n - 5, 9] = np.nan
-    with pytest.raises(sg.SograndError) as e:
-        decoder.decode(llr)
-    assert e.value.code == sg.ENONFINITE
 
 
-@pytest.mark.parametrize("bad", [np.nan, np.inf, -np.inf])
-def test_nonfinite_input_is_an_error(decoder, bad):
-    sg = _sg()
-    llr = sg.gen_llr(1, 3, ebn0_db=2.0)
-    llr[1, 77] = bad
-    for backend in (0, 1):
-        decoder.set_params(backend=backend)
-        with pytest.raises(sg.SograndError) as e:
-            decoder.decode(llr)
-        assert e.value.code == sg.ENONFINITE
+def _same_or_close(a, b) -> bool:
+    """Equal where the reference's value is not finite (NaN to NaN, inf to the
+    same inf), within SOFT_RTOL elsewhere."""
+    a = np.asarray(a)
+    b = np.asarray(b)
+    fin = np.isfinite(b)
+    same_nonfinite = np.array_equal(a[~fin], b[~fin], equal_nan=True)
+    return same_nonfinite and soft_close(a[fin], b[fin])
+
+
+def test_nonfinite_lines_match_reference(decoder):
+    # lines with +-inf entries, with a NaN, all infinite (recorded from the
+    # reference by tests/golden/make_golden_nonfinite.py)
+    g = golden("nonfinite.npz")
+    decoder.set_params(backend=1)
+    la, le, ng = decoder.bitso(g["lines"])
+    assert (ng == g["lines_N_guess"]).all(), np.flatnonzero(ng != g["lines_N_guess"])[:10]
+    assert _same_or_close(la, g["lines_L_APP"])
+    assert _same_or_close(le, g["lines_L_E"])
+
+
+@pytest.mark.parametrize("backend", [0, 1], ids=["plain", "persistent"])
+def test_nonfinite_cubes_match_reference(decoder, backend):
+    # cubes with a few +-inf entries (which the reference's channel can emit)
+    # and a NaN: the reference runs all 30 iterations on them, as the NaN of
+    # inf - inf spreads; bits, counts and final L_APP as it computes them
+    sys.path.insert(0, GOLDEN)
+    from make_golden_nonfinite import cube_llr
+
+    g = golden("nonfinite.npz")
+    decoder.set_params(backend=backend)
+    for snr in ("1.5", "2.0"):
+        llr = cube_llr(snr)
+        r = decoder.decode(llr, soft=True)
+        assert (r.bits == g[f"bits_{snr}"]).all(), snr
+        for f in ("half_iters", "NG", "NG_p"):
+            assert (r.stats[f] == g[f"{f}_{snr}"]).all(), (snr, f)
+        assert _same_or_close(r.soft, g[f"soft_{snr}"]), snr
 
 
 def test_empty_batch(decoder):

@@ -87,8 +87,10 @@ def test_admission_and_errors(decoder):
         assert srv.submit(2, bad, bits_b) == sg.OK
         srv.drain()
         res = {c.id: c for c in srv.poll()}
-        assert res[1].status == sg.OK and res[2].status == sg.ENONFINITE
+        # a frame with a NaN decodes as the reference's arithmetic decodes it
+        assert res[1].status == sg.OK and res[2].status == sg.OK
         assert (bits_g == decoder.decode(good).bits).all()
+        assert (bits_b == decoder.decode(bad).bits).all()
         # admission control: more waiting codewords than max_pending_cw is refused
         frames = [(sg.gen_llr(7, 16, ebn0_db=1.0, cw_offset=16 * k), np.zeros((16, 64), np.uint8))
                   for k in range(40)]

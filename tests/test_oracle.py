@@ -7,6 +7,7 @@ textbook definition.
 """
 import itertools
 import json
+import sys
 import os
 
 import numpy as np
@@ -193,11 +194,24 @@ def test_cubes_match_reference_binary(oracle, lib, snr):
     assert (soft[:ns] == g["soft"]).all()
 
 
-def test_oracle_rejects_nonfinite(oracle):
-    x = np.ones(16)
-    x[3] = np.nan
-    with pytest.raises(ValueError):
-        oracle.siso(x)
+def test_oracle_nonfinite_matches_reference(oracle):
+    # +-inf and NaN inputs as the reference processes them (fixtures of
+    # tests/golden/make_golden_nonfinite.py): lines and whole decodes bit for bit
+    sys.path.insert(0, GOLDEN)
+    from make_golden_nonfinite import CUBES, cube_llr
+
+    g = golden("nonfinite.npz")
+    for i, x in enumerate(g["lines"]):
+        la, le, tr = oracle.siso(x)
+        assert tr.T == g["lines_N_guess"][i], i
+        assert np.array_equal(la, g["lines_L_APP"][i], equal_nan=True), i
+        assert np.array_equal(le, g["lines_L_E"][i], equal_nan=True), i
+    for snr in CUBES:
+        bits, st, soft = oracle.decode(cube_llr(snr), soft=True)
+        assert (bits == g[f"bits_{snr}"]).all()
+        for f in ("half_iters", "NG", "NG_p"):
+            assert (st[f] == g[f"{f}_{snr}"]).all(), f
+        assert np.array_equal(soft, g[f"soft_{snr}"], equal_nan=True)
 
 
 def test_oracle_epsilon_variant_matches_reference_variant(oracle):
