@@ -83,34 +83,6 @@ SG_HD inline uint32_t encode_row(uint32_t m)
 }
 
 // ---------------------------------------------------------------------------
-// Batcher odd-even merge sort network for 16 keys (63 comparators), generated
-// at compile time. Verified exhaustively (0-1 principle) by the unit tests.
-// ---------------------------------------------------------------------------
-struct SortNet {
-    int n;
-    int8_t lo[64], hi[64];
-};
-
-constexpr SortNet make_sortnet16()
-{
-    SortNet s{};
-    s.n = 0;
-    const int n = 16;
-    for (int p = 1; p < n; p <<= 1)
-        for (int k = p; k >= 1; k >>= 1)
-            for (int j = k % p; j + k < n; j += 2 * k)
-                for (int i = 0; i < k && i < n - j - k; ++i)
-                    if ((i + j) / (2 * p) == (i + j + k) / (2 * p)) {
-                        s.lo[s.n] = (int8_t)(i + j);
-                        s.hi[s.n] = (int8_t)(i + j + k);
-                        ++s.n;
-                    }
-    return s;
-}
-constexpr SortNet kSortNet = make_sortnet16();
-static_assert(kSortNet.n == 63, "odd-even merge sort of 16 keys has 63 comparators");
-
-// ---------------------------------------------------------------------------
 // Error-pattern schedule geometry. A pattern is a 16-bit mask over sorted
 // positions (bit b = b-th least reliable bit); w = popcount, S = sum of (b+1),
 // logistic weight wt = IC*w + S. The reference visits wt = IC+1, IC+2, ...,

@@ -194,6 +194,28 @@ def test_finite_tmax_matches_oracle(decoder, oracle, tmax):
         assert (r2.bits == r.bits).all() and (r2.stats == r.stats).all()
 
 
+@pytest.mark.parametrize("shape", [
+    {"wave_slots": 1, "lane_batch": 8},  # what the autotuner picks on the uneven mix
+    {"wave_slots": 3, "lane_batch": 16, "step_len": 256},
+    {"wave_slots": 4, "lane_batch": 1, "step_len": 16, "waves_per_cu": 4},
+], ids=["tuned", "s3", "s4"])
+def test_launch_shapes_match_oracle(decoder, oracle, shape):
+    # the persistent kernel's shape parameters change only speed: per-codeword
+    # Eb/N0 over the uneven configuration's 0.5-3.5 dB range
+    sg = _sg()
+    ebn0 = np.random.default_rng(8).uniform(0.5, 3.5, 160)
+    llr = sg.gen_llr(8080, 160, ebn0_db=ebn0)
+    bits, st, soft = oracle.decode(llr, soft=True)
+    decoder.set_params(backend=1, **shape)
+    try:
+        r = decoder.decode(llr, soft=True)
+    finally:
+        decoder.set_params(wave_slots=0, lane_batch=0, step_len=0, waves_per_cu=0)
+    assert (r.bits == bits).all()
+    assert (r.stats == st).all()
+    assert soft_close(r.soft, soft), soft_err(r.soft, soft)
+
+
 def test_generator_device_equals_host(decoder):
     sg = _sg()
     from sogrand_amd import hip

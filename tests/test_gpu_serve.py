@@ -40,11 +40,13 @@ def test_frames_identical_alone_and_in_the_mix(decoder, precision):
     from sogrand_amd import hip
 
     decoder.set_params(backend=sg.BACKEND_PERSISTENT, precision=precision)
-    frames = _frames(2024 + precision, 40, 48)
+    # frames of 1-256 codewords, as the serving configuration draws them
+    frames = _frames(2024 + precision, 24, 256)
     alone = [decoder.decode(f, soft=True) for f in frames]
-    _say("frames:", len(frames), "codewords:", sum(len(f) for f in frames))
-    # small batches and four slots: many batches, several in flight at once
-    with sg.Server(decoder, hbm_budget=1 << 30, max_inflight=4, max_batch_cw=96,
+    _say("frames:", len(frames), "codewords:", sum(len(f) for f in frames),
+         "largest:", max(len(f) for f in frames))
+    # batches of at most 384 codewords and four slots: several in flight at once
+    with sg.Server(decoder, hbm_budget=2 << 30, max_inflight=4, max_batch_cw=384,
                    with_soft=True) as srv:
         outs = []
         for k, f in enumerate(frames):
