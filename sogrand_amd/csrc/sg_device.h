@@ -43,14 +43,32 @@ struct DecodeCfg {
 
 // Entries of the LDS table: the exp table, then one entry {eps, thres} that the
 // list insertion reads where it needs them (as a broadcast load, which keeps
-// the two doubles out of the scalar registers the search loop runs short of).
-constexpr int kLdsTableEntries = kExpTableEntries + 1;
+// the two doubles out of the scalar registers the search loop runs short of),
+// then the log's polynomial coefficients (kLogCoef, two per entry).
+constexpr int kLdsLogCoefEntry = kExpTableEntries + 1;
+constexpr int kLdsTableEntries = kLdsLogCoefEntry + kLogCoefs / 2;
 
 // Fill the LDS table (first kLdsTableEntries threads) - the caller synchronises.
 __device__ __forceinline__ void load_exp_table(ExpEntry *lds, int tid, double eps, double thres)
 {
-    if (tid < kExpTableEntries) lds[tid] = exp_table_entry(tid);
-    else if (tid == kExpTableEntries) lds[tid] = ExpEntry{eps, thres};
+    if (tid < kExpTableEntries) {
+        lds[tid] = exp_table_entry(tid);
+    } else if (tid == kExpTableEntries) {
+        lds[tid] = ExpEntry{eps, thres};
+    } else if (tid < kLdsTableEntries) {
+        const int j = 2 * (tid - kLdsLogCoefEntry);
+        lds[tid] = ExpEntry{kLogCoef[j], kLogCoef[j + 1]};
+    }
+}
+
+// The log's coefficients in the LDS table, read where the log runs: the table
+// is never written after it is filled, so without the opaque zero offset the
+// compiler would hoist all fifteen loads into thirty kernel-wide registers.
+__device__ __forceinline__ const double *log_coefs(const DecodeCfg &cfg)
+{
+    int z = 0;
+    asm volatile("" : "+v"(z));
+    return reinterpret_cast<const double *>(cfg.exptab + kLdsLogCoefEntry + z);
 }
 
 // exp(-pm) for pm >= 0, the probability weight of one guess (:626, :672, :712).
@@ -157,7 +175,7 @@ __device__ __forceinline__ double jaclog(double x, double lim, const DecodeCfg &
 {
     if (x > lim) return x;
     if (x < -lim) return 0.0;
-    return log_pos(1.0 + exp_main(x, cfg.exptab), cfg.logtab); // |x| <= lim < 512
+    return log_pos(1.0 + exp_main(x, cfg.exptab), cfg.logtab, log_coefs(cfg)); // |x| <= lim < 512
 }
 
 
@@ -841,7 +859,9 @@ __device__ __forceinline__ void line_search(const LDS &S, uint32_t hd,
 __device__ __forceinline__ double soft_app(const ListEntries &ls, int nl, double pnotin, int i,
                                            double x, const DecodeCfg &cfg)
 {
-    const double eps = cfg.eps, eps_log = cfg.eps_log;
+    // eps from the LDS table entry, read here: 1 - eps is then formed where it
+    // is used rather than kept live (and spilled) for the whole kernel
+    const double eps = cfg.exptab[kExpTableEntries].a, eps_log = cfg.eps_log;
     double pp1 = 1.0 / (1.0 + exp_any(x, cfg.exptab));
     double pp0 = 1.0 - pp1;
     pp1 = fmin(fmax(pp1, eps), 1.0 - eps);
@@ -856,7 +876,8 @@ __device__ __forceinline__ double soft_app(const ListEntries &ls, int nl, double
         }
     p0 += pnotin * pp0;
     p1 += pnotin * pp1;
-    return log_pos(fmax(p0, eps_log), cfg.logtab) - log_pos(fmax(p1, eps_log), cfg.logtab);
+    return log_pos(fmax(p0, eps_log), cfg.logtab, log_coefs(cfg)) -
+           log_pos(fmax(p1, eps_log), cfg.logtab, log_coefs(cfg));
 }
 
 } // namespace sg

@@ -163,7 +163,6 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))
     load_exp_table(s_exp, threadIdx.x, job.eps, job.thres);
     __syncthreads(); // the only workgroup barrier: waves run independently from here
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const uint64_t lt_mask = (1ull << lane) - 1ull;
     Rec *rec_d = &s_d[wave][0];
     uint16_t *rec_h = &s_h[wave][0];
     const LineLdsT<Rec, 64> L(rec_d, rec_h, lane);
@@ -241,7 +240,12 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))
                 // retire: info bits from the hard decision (:267-280), statistics
                 const int h = (lane & 7) + 16 * (lane >> 3);
                 const uint32_t byte = rev8((hb[h >> 1] >> (16 * (h & 1))) & 0xffu);
-                job.info[(size_t)sl.cube[s] * kInfoBytes + lane] = (uint8_t)byte;
+                // the lane offset is made opaque here so that the compiler does
+                // not keep job.info + lane live (in spilled registers) for the
+                // whole kernel
+                int lo = lane;
+                asm volatile("" : "+v"(lo));
+                job.info[(size_t)sl.cube[s] * kInfoBytes + (size_t)lo] = (uint8_t)byte;
                 if (lane == 0) {
                     if (job.stats) {
                         CwStatsOut *o = reinterpret_cast<CwStatsOut *>(job.stats) + sl.cube[s];
@@ -383,7 +387,9 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))
                 const uint32_t left = 256u - sl.nxt_line[s];
                 const uint32_t ni = (uint32_t)__popcll(idle);
                 const uint32_t k = ni < left ? ni : left;
-                const uint32_t rank = (uint32_t)__popcll(idle & lt_mask);
+                // idle lanes below this one (v_mbcnt: no lane mask kept live)
+                const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
+                    (uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
                 if (state == kIdle && rank < k) {
                     state = kNew;
                     my_slot = s;
