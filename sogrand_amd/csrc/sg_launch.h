@@ -18,8 +18,8 @@ struct CubeJob {
     uint8_t *info;            // n_cw x 64 (device)
     double *soft;             // n_cw x 4096 or null (device)
     uint32_t *stats;          // n_cw x 6 u32 {half, conv, NG lo, NG hi, NGp lo, NGp hi} or null
-    double *le_scratch;       // 4096-f64 cubes: one per plain workgroup, two per
-                              // persistent wave slot (kPersistentSlotScratch)
+    double *le_scratch;       // 4096-f64 cubes: one per plain workgroup, three
+                              // per persistent wave slot (kPersistentSlotScratch)
     uint32_t *queue;          // persistent: work counter (device), zeroed by the launcher
     size_t n_cw;
     const uint16_t *sched;
@@ -53,9 +53,9 @@ hipError_t noalloc_scratch_init(int device);
 // Resident workgroups per CU of the persistent kernel (occupancy API).
 int persistent_blocks_per_cu(int slots, bool finite, int arith);
 constexpr int kPersistentWaves = 4;
-// 4096-f64 scratch cubes per persistent wave slot: L_E and the current phase's
-// line inputs (sg_persistent.hip)
-constexpr int kPersistentSlotScratch = 2;
+// 4096-f64 scratch cubes per persistent wave slot: two L_E cubes used in turn
+// and the current phase's line inputs (sg_persistent.hip)
+constexpr int kPersistentSlotScratch = 3;
 
 // Line-level SISO (SOGRAND_bitSO on many lines).
 hipError_t launch_lines(const double *llr, size_t n_lines, double *la, double *le,

@@ -35,26 +35,62 @@ struct CwStatsOut {
     uint64_t NG_p;
 };
 
-// Device scratch per cube slot, in cubes of 4096 doubles: L_E in natural order,
-// then the inputs x = L_ch + alpha L_E of the current phase's lines, line-major
-// (16 line + i), written by the line set-up and read back by the soft output of
-// the same line, which then reads 16 consecutive doubles instead of two strided
-// sets (the line's own L_E positions are not written in between).
+// Device scratch per cube slot, in cubes of 4096 doubles: two L_E cubes used in
+// turn, then the inputs x = L_ch + alpha L_E of the current phase's lines.
+// The set-up stores x line-major (16 line + i: a line's 16 values are one
+// 128-byte row), where the soft output of the same line reads it back. The L_E
+// of a phase goes to one L_E cube and the next phase reads it from there while
+// writing the other (phase_geom gives both layouts).
 constexpr int kSlotScratch = kPersistentSlotScratch;
+static_assert(kSlotScratch == 3, "two L_E cubes and the line inputs");
 
-__device__ __forceinline__ void axis_geom(int ax, int line, int &base, int &stride)
+// Where the lines of a phase sit. Every 16 lanes of an instruction work on
+// the 16 positions of one line, four lines an instruction, and lines finish in
+// no particular order. Loads want four consecutive lines to read 32 contiguous
+// bytes at each position; stores want a finished line's 16 values in one
+// 128-byte row. Per axis (a0 + 16 a1 + 256 a2 natural, lo = line & 15,
+// hi = line >> 4):
+//   axis 0: lines a2 = lo, a1 = hi (natural rows); reads axis 2's rows at
+//           lo + 256 hi + 16 i; writes its rows, 16 line + i
+//   axis 1: lines a0 = lo, a2 = hi; reads axis 0's rows at lo + 16 hi + 256 i;
+//           writes natural order (single doubles 128 bytes apart)
+//   axis 2: lines a0 = lo, a1 = hi; reads natural order; writes its rows
+// The channel LLRs are read in natural order: whole rows on axis 0, 32 bytes
+// per four lines on axes 1 and 2. Only axis 1's stores are scattered: writing
+// rows there would leave axis 2 either the L_E or the channel loads at single
+// doubles 128 bytes apart, which the counters show to cost more.
+struct PhaseGeom {
+    int base, stride;   // natural positions base + stride i
+    int rbase, rstride; // L_E read: rbase + rstride i in the previous phase's cube
+    int wbase, wstride; // L_E write: wbase + wstride i in this phase's cube
+};
+__device__ __forceinline__ PhaseGeom phase_geom(int ax, int line)
 {
-    const int x = line & 15, y = line >> 4;
+    const int lo = line & 15, hi = line >> 4;
+    PhaseGeom g;
     if (ax == 0) {
-        base = 16 * x + 256 * y;
-        stride = 1;
+        g.base = 256 * lo + 16 * hi;
+        g.stride = 1;
+        g.rbase = lo + 256 * hi;
+        g.rstride = 16;
+        g.wbase = 16 * line;
+        g.wstride = 1;
     } else if (ax == 1) {
-        base = x + 256 * y;
-        stride = 16;
+        g.base = lo + 256 * hi;
+        g.stride = 16;
+        g.rbase = lo + 16 * hi;
+        g.rstride = 256;
+        g.wbase = g.base;
+        g.wstride = g.stride;
     } else {
-        base = x + 16 * y;
-        stride = 256;
+        g.base = lo + 16 * hi;
+        g.stride = 256;
+        g.rbase = g.base;
+        g.rstride = g.stride;
+        g.wbase = 16 * line;
+        g.wstride = 1;
     }
+    return g;
 }
 
 // Early-termination test of one 16x16 bit plane streamed word by word: word j
@@ -195,16 +231,32 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))
     ss.t = ~0u; // no line yet: t + 1 = 0 guesses
     ss.done = true;
 
-    // geometry of (slot, line) as seen by any lane
-    auto line_geom = [&](int s, int line, int &base, int &stride, const double *&lch, double *&le,
-                         double *&soft, bool &first) {
+    // geometry of (slot, line) as seen by any lane: positions (phase_geom),
+    // the L_E cubes this phase writes (le_wr) and reads (le_rd), the line
+    // inputs (xs, line-major)
+    struct LineGeom {
+        PhaseGeom g;
+        const double *lch;
+        const double *le_rd;
+        double *le_wr, *xs, *soft;
+        bool first;
+    };
+    auto line_geom = [&](int s, int line) {
+        LineGeom lg;
         const uint32_t c = geom[2 * s], g = geom[2 * s + 1];
         const uint32_t ax = g & 0xffu, it = g >> 8;
-        axis_geom((int)ax, line, base, stride);
-        lch = job.llr + (size_t)c * kCube;
-        le = job.le_scratch + (wave_id * S + (size_t)s) * (kSlotScratch * kCube);
-        soft = job.soft ? job.soft + (size_t)c * kCube : nullptr;
-        first = (it == 1 && ax == 0);
+        lg.g = phase_geom((int)ax, line);
+        lg.lch = job.llr + (size_t)c * kCube;
+        double *sc = job.le_scratch + (wave_id * S + (size_t)s) * (kSlotScratch * kCube);
+        // phases alternate the two L_E cubes: phase 3 (it - 1) + ax + 1 writes
+        // cube (it + ax) & 1 and reads the other one
+        const uint32_t wb = (it + ax) & 1u;
+        lg.le_wr = sc + (size_t)wb * kCube;
+        lg.le_rd = sc + (size_t)(wb ^ 1u) * kCube;
+        lg.xs = sc + 2 * kCube;
+        lg.soft = job.soft ? job.soft + (size_t)c * kCube : nullptr;
+        lg.first = (it == 1 && ax == 0);
+        return lg;
     };
 
     for (;;) {
@@ -353,13 +405,9 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))
                     ent.cw[j] = (uint32_t)__shfl((int)my_cw, (lane & ~15) + j, 64);
                 }
                 if (owner >= 0) {
-                    int base, stride;
-                    const double *lch;
-                    double *le, *soft;
-                    bool first;
-                    line_geom(o_slot, o_line, base, stride, lch, le, soft, first);
-                    const int p = base + i * stride;
-                    const double xin = le[kCube + 16 * o_line + i]; // the set-up's x
+                    const LineGeom lg = line_geom(o_slot, o_line);
+                    const int p = lg.g.base + i * lg.g.stride;
+                    const double xin = lg.xs[16 * o_line + i]; // the set-up's x
                     double la, lenew;
                     if (ol.nl == 0) { // empty list: L_APP = llr, L_E = 0 (:537-539)
                         la = xin;
@@ -368,8 +416,8 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))
                         la = soft_app(ent, ol.nl, fmax(ol.pnl, cfg.eps), i, xin, cfg);
                         lenew = la - xin;
                     }
-                    le[p] = lenew;
-                    if (soft) soft[p] = la;
+                    lg.le_wr[lg.g.wbase + i * lg.g.wstride] = lenew;
+                    if (lg.soft) lg.soft[p] = la;
                     if (!(la > 0.0)) atomicOr(&hd_bits[128 * o_slot + (p >> 5)], 1u << (p & 31));
                 }
                 if (lane == o[0] || lane == o[1] || lane == o[2] || lane == o[3]) {
@@ -414,16 +462,12 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))
                 Rec key = 0;
                 uint32_t rank = 0;
                 if (owner >= 0) {
-                    int base, stride;
-                    const double *lch;
-                    double *le, *soft;
-                    bool first;
-                    line_geom(o_slot, o_line, base, stride, lch, le, soft, first);
-                    const int p = base + i * stride;
-                    const double e = first ? 0.0 : le[p];
+                    const LineGeom lg = line_geom(o_slot, o_line);
+                    const int p = lg.g.base + i * lg.g.stride;
+                    const double e = lg.first ? 0.0 : lg.le_rd[lg.g.rbase + i * lg.g.rstride];
                     const double tt = job.alpha * e; // L_A = alpha L_E, then + L_channel (:147-155)
-                    const double x = lch[p] + tt;
-                    le[kCube + 16 * o_line + i] = x; // read back by the soft output
+                    const double x = lg.lch[p] + tt;
+                    lg.xs[16 * o_line + i] = x; // read back by the soft output
                     neg = !(x > 0.0); // HardDec (:758)
                     ai = fabs(x);
                     key = (Rec)ai;

@@ -315,9 +315,9 @@ int sogrand_server_create(sogrand_server **out, sogrand_ctx *ctx, const sogrand_
         s->p = ctx->p;
     }
     s->cfg = c;
-    // per-codeword bytes of a slot: LLRs, L_E scratch (up to two cubes per
-    // codeword for the persistent kernel), optional soft output, bits,
-    // statistics
+    // per-codeword bytes of a slot: LLRs, L_E scratch (up to
+    // kPersistentSlotScratch cubes per codeword for the persistent kernel),
+    // optional soft output, bits, statistics
     const size_t per_cw = (1 + sg::kPersistentSlotScratch) * sg::kCube * sizeof(double) +
                           (c.with_soft ? sg::kCube * sizeof(double) : 0) + sg::kInfoBytes +
                           sizeof(sogrand_cw_stats);
