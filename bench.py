@@ -529,7 +529,10 @@ def kernel_source_hash() -> str:
 
     from sogrand_amd import build as b
 
-    h = hashlib.sha256(" ".join(b.COMMON).encode())
+    # flags with the tree's own paths taken out, so the hash names the same
+    # build wherever the tree sits
+    flags = [f.replace(ROOT, ".") for f in b.COMMON]
+    h = hashlib.sha256(" ".join(flags).encode())
     for name in sorted(os.listdir(b.CSRC)):
         if name.endswith((".hip", ".h", ".cpp")):
             with open(os.path.join(b.CSRC, name), "rb") as f:

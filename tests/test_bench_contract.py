@@ -69,3 +69,23 @@ def test_traffic_record_matches_the_sources_or_is_refused():
 def test_cpu_baseline_uses_every_allowed_core():
     bench = _bench()
     assert bench.host_cores() == len(os.sched_getaffinity(0))
+
+
+def test_source_hash_does_not_depend_on_where_the_tree_sits():
+    # a copy of the tree elsewhere (as on another machine) hashes the same
+    import shutil
+    import tempfile
+
+    bench = _bench()
+    here = bench.kernel_source_hash()
+    with tempfile.TemporaryDirectory() as d:
+        dst = os.path.join(d, "copy")
+        os.makedirs(dst)
+        shutil.copy(os.path.join(ROOT, "bench.py"), dst)
+        shutil.copytree(os.path.join(ROOT, "sogrand_amd"), os.path.join(dst, "sogrand_amd"),
+                        ignore=shutil.ignore_patterns("_obj", "*.so", "cubic_decoder"))
+        r = subprocess.run([sys.executable, "-c",
+                            "import bench; print(bench.kernel_source_hash())"],
+                           cwd=dst, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        assert r.stdout.strip() == here
