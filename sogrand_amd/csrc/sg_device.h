@@ -94,8 +94,9 @@ __device__ __forceinline__ double exp_neg(double pm, const ExpEntry *tab)
 // again with exact weights (chain_exact, search_exact).
 __device__ __forceinline__ double exp_neg_loop(double pm, const ExpEntry *tab)
 {
+    // the cap from a scalar register (a vector one would be rebuilt per use)
     double pc;
-    asm("v_min_f64 %0, %1, %2" : "=v"(pc) : "v"(pm), "v"(708.0));
+    asm("v_min_f64 %0, %1, %2" : "=v"(pc) : "v"(pm), "s"(708.0));
     return exp_main(-pc, tab);
 }
 
@@ -645,6 +646,16 @@ __device__ __forceinline__ void pair_account(SearchState &ss, LineList &ls, cons
     ss.done = stop;
 }
 
+// a ^ b ^ c in one instruction: gfx950's three-input bit operation with the
+// parity truth table 0x96 (as C, the H-column XORs of 16-bit loads come out as
+// two XORs with a word select)
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c)
+{
+    uint32_t r;
+    asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0x96" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+
 template <int I> struct IntC {
     static constexpr int value = I;
 };
@@ -663,12 +674,13 @@ template <int I> __device__ __forceinline__ uint32_t pos_field(uint32_t w)
 // position words w1 and w2 at schedule positions t and t + 1. Each lane
 // carries two independent path-metric and exp chains; the guesses are still
 // accounted strictly in schedule order (the second is dropped if the first
-// ends the search). The fields of both words are read two at a time; the
-// rounds past the first take a wave-uniform branch, so a round runs only if
-// some lane's pattern has that many set bits, and a lane whose pattern has
-// fewer reads the null record (row 0), which leaves both sums unchanged. Bits
-// past the sixth (rare) come from the mask. The path metrics add the
-// increments in ascending sorted position, as getPM does (:765).
+// ends the search). The fields of both words are read two at a time in three
+// rounds that every lane runs (some lane of a wave nearly always has a pattern
+// of five or six bits, so a wave-uniform skip of the later rounds measured
+// slower: it kept the rounds' loads from overlapping); a lane whose pattern
+// has fewer bits reads the null record (row 0), which leaves both sums
+// unchanged. Bits past the sixth (rare) come from the mask. The path metrics
+// add the increments in ascending sorted position, as getPM does (:765).
 //
 // Under a guess budget the guard T < Tmax runs once per weight level (:652),
 // before the first pattern of each level (the words' level bit), with T = t + 1.
@@ -702,8 +714,8 @@ __device__ __forceinline__ void search_pair(SearchState &ss, LineList &ls, const
     {
         const uint32_t a1 = pos_field<0>(w1), b1 = pos_field<1>(w1);
         const uint32_t a2 = pos_field<0>(w2), b2 = pos_field<1>(w2);
-        syn1 = s0 ^ (uint32_t)S.hr(a1) ^ (uint32_t)S.hr(b1);
-        syn2 = s0 ^ (uint32_t)S.hr(a2) ^ (uint32_t)S.hr(b2);
+        syn1 = xor3(s0, (uint32_t)S.hr(a1), (uint32_t)S.hr(b1));
+        syn2 = xor3(s0, (uint32_t)S.hr(a2), (uint32_t)S.hr(b2));
         if constexpr (kPmLoop) {
             pm1 = (pm_base + S.dr(a1)) + S.dr(b1);
             pm2 = (pm_base + S.dr(a2)) + S.dr(b2);
@@ -713,18 +725,15 @@ __device__ __forceinline__ void search_pair(SearchState &ss, LineList &ls, const
         constexpr int I = decltype(i0)::value;
         const uint32_t a1 = pos_field<I>(w1), b1 = pos_field<I + 1>(w1);
         const uint32_t a2 = pos_field<I>(w2), b2 = pos_field<I + 1>(w2);
-        syn1 ^= (uint32_t)S.hr(a1) ^ (uint32_t)S.hr(b1);
-        syn2 ^= (uint32_t)S.hr(a2) ^ (uint32_t)S.hr(b2);
+        syn1 = xor3(syn1, (uint32_t)S.hr(a1), (uint32_t)S.hr(b1));
+        syn2 = xor3(syn2, (uint32_t)S.hr(a2), (uint32_t)S.hr(b2));
         if constexpr (kPmLoop) {
             pm1 = (pm1 + S.dr(a1)) + S.dr(b1);
             pm2 = (pm2 + S.dr(a2)) + S.dr(b2);
         }
     };
-    // fields 2.. (and the overflow bit) sit above bit 10, fields 4.. above bit 20
-    if (__ballot(((w1 | w2) >> 11) != 0u) != 0ull) {
-        round(IntC<2>{});
-        if (__ballot(((w1 | w2) >> 21) != 0u) != 0ull) round(IntC<4>{});
-    }
+    round(IntC<2>{});
+    round(IntC<4>{});
     const int tbl = ic < kSchedGenericIc ? ic : kSchedGenericIc;
     const uint16_t *masks = cfg.sched + (size_t)tbl * kSchedStride;
     if (__ballot(((w1 | w2) & kPosMore) != 0u) != 0ull) {
