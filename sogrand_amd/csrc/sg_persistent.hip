@@ -166,27 +166,6 @@ __device__ __forceinline__ uint32_t xor16(uint32_t v)
     return v;
 }
 
-// Lane 16q + I's double in every lane of the group 16q .. 16q + 15 (DPP
-// row_newbcast on both words).
-template <int I> __device__ __forceinline__ double row_bcast_d(double v)
-{
-    const uint64_t u = double_to_bits(v);
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, 0x150 + I, 0xF, 0xF, false);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), 0x150 + I, 0xF, 0xF, false);
-    return bits_to_double(((uint64_t)hi << 32) | lo);
-}
-
-// 0.0 + v_0 + v_1 + ... + v_15 over the lanes of each group of 16, added in
-// lane order (each addition rounded in turn), in every lane of the group.
-template <int I = 0> __device__ __forceinline__ double row_ordered_sum(double v, double acc = 0.0)
-{
-    if constexpr (I == 16) {
-        return acc;
-    } else {
-        return row_ordered_sum<I + 1>(v, acc + row_bcast_d<I>(v));
-    }
-}
-
 // OR of v over each group of 16 lanes, in every lane of the group (the
 // pattern of xor16).
 __device__ __forceinline__ uint32_t or16(uint32_t v)
@@ -212,6 +191,7 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))
     __shared__ uint16_t s_h[kWaves][17 * 64];
     __shared__ uint32_t s_hd[kWaves][S][128];
     __shared__ uint32_t s_cnt[kWaves][S][3]; // lines done, phase guess sum, phase max
+    __shared__ double s_stage_jn[kWaves][64];
     __shared__ double s_stage_a[kWaves][4][2]; // per sub-round line: |x| of ranks 0 and 7
     __shared__ uint32_t s_geom[kWaves][S][2];  // per slot: cube, axis | iteration << 8
     __shared__ ExpEntry s_exp[kLdsTableEntries];
@@ -226,6 +206,7 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))
     const size_t wave_id = (size_t)blockIdx.x * kWaves + wave;
     uint32_t *hd_bits = &s_hd[wave][0][0];
     uint32_t *cnt = &s_cnt[wave][0][0];
+    double *stage_jn = &s_stage_jn[wave][0];
     uint32_t *geom = &s_geom[wave][0][0];
     double *stage_a = &s_stage_a[wave][0][0];
 
@@ -477,7 +458,7 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))
                 const int o_line = __shfl(my_line, src, 64);
                 const LineLdsT<Rec, 64> R(rec_d, rec_h, src); // the owner's records
                 bool tie = false, neg = false;
-                double ai = 0.0, delta = 0.0, jn = 0.0;
+                double ai = 0.0, delta = 0.0;
                 Rec key = 0;
                 uint32_t rank = 0;
                 if (owner >= 0) {
@@ -511,8 +492,9 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))
 #pragma unroll 4
                         for (int j = 0; j < 16; ++j) rank += R.d(j) < key ? 1u : 0u;
                     }
-                    jn = jaclog(-ai, P::kJacLim, cfg);
+                    const double jn = jaclog(-ai, P::kJacLim, cfg);
                     delta = jaclog(ai, P::kJacLim, cfg) - jn; // getPM's increment (:765)
+                    stage_jn[lane] = jn;
                     // the two order statistics of IC (a tie line's IC is taken
                     // from its sorted records instead)
                     if (rank == 0) stage_a[2 * q] = (double)key;
@@ -530,15 +512,14 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(4))
                               : lane == o[3] ? 3 : -1;
                 // the owner of line q takes the syndrome from the line's lanes
                 const uint32_t syn_owner = (uint32_t)__shfl((int)syn16, 16 * (myq >= 0 ? myq : 0), 64);
-                // getPM_HD (:617): the line's 16 JacLog(-|x_i|) summed in natural
-                // order over its lanes, then handed to the owner
-                const double pm_hd_owner = shfl_d(row_ordered_sum(jn), 16 * (myq >= 0 ? myq : 0));
                 bool line_special = false;
                 if (myq >= 0) {
                     line_special = ((special >> (16 * myq)) & 0xffffull) != 0;
                     hd = (uint32_t)(negm >> (16 * myq)) & 0xffffu;
                     s0 = syn_owner;
-                    pm_hd = pm_hd_owner;
+                    pm_hd = 0.0;
+#pragma unroll
+                    for (int j = 0; j < 16; ++j) pm_hd += stage_jn[16 * myq + j]; // getPM_HD (:617)
                     // IC (:619-622); a tie line's is taken from its sorted records
                     ic = intercept(stage_a[2 * myq], stage_a[2 * myq + 1], P::kLegacy);
                 }
