@@ -675,11 +675,11 @@ template <int I> __device__ __forceinline__ uint32_t pos_field(uint32_t w)
 // carries two independent path-metric and exp chains; the guesses are still
 // accounted strictly in schedule order (the second is dropped if the first
 // ends the search). The fields of both words are read two at a time in three
-// rounds that every lane runs (some lane of a wave nearly always has a pattern
-// of five or six bits, so a wave-uniform skip of the later rounds measured
-// slower: it kept the rounds' loads from overlapping); a lane whose pattern
-// has fewer bits reads the null record (row 0), which leaves both sums
-// unchanged. Bits past the sixth (rare) come from the mask. The path metrics
+// rounds that every lane runs in double precision (some lane of a wave nearly
+// always has a pattern of five or six bits, so a wave-uniform skip of the later
+// rounds measured slower: it kept the rounds' loads from overlapping); a lane
+// whose pattern has fewer bits reads the null record (row 0), which leaves both
+// sums unchanged. Bits past the sixth (rare) come from the mask. The path metrics
 // add the increments in ascending sorted position, as getPM does (:765).
 //
 // Under a guess budget the guard T < Tmax runs once per weight level (:652),
@@ -732,8 +732,18 @@ __device__ __forceinline__ void search_pair(SearchState &ss, LineList &ls, const
             pm2 = (pm2 + S.dr(a2)) + S.dr(b2);
         }
     };
-    round(IntC<2>{});
-    round(IntC<4>{});
+    if constexpr (P::kF32) {
+        // single-precision rounds are cheaper, and a wave-uniform skip of the
+        // later rounds measured 5 % faster there (fields 2.. and the overflow
+        // bit sit above bit 10, fields 4.. above bit 20)
+        if (__ballot(((w1 | w2) >> 11) != 0u) != 0ull) {
+            round(IntC<2>{});
+            if (__ballot(((w1 | w2) >> 21) != 0u) != 0ull) round(IntC<4>{});
+        }
+    } else {
+        round(IntC<2>{});
+        round(IntC<4>{});
+    }
     const int tbl = ic < kSchedGenericIc ? ic : kSchedGenericIc;
     const uint16_t *masks = cfg.sched + (size_t)tbl * kSchedStride;
     if (__ballot(((w1 | w2) & kPosMore) != 0u) != 0ull) {

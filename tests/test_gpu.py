@@ -195,6 +195,26 @@ def test_finite_tmax_matches_oracle(decoder, oracle, tmax):
         assert (r2.bits == r.bits).all() and (r2.stats == r.stats).all()
 
 
+@pytest.mark.parametrize("L", [1, 2])
+def test_smaller_lists_match_oracle(decoder, oracle, L):
+    # list sizes below the reference's L = 3 (the parameter is a runtime one):
+    # the stopping rule fires at a list of L entries and the soft output sums L
+    # entries; per-codeword Eb/N0 over 0.5-3.5 dB, both backends
+    sg = _sg()
+    ebn0 = np.random.default_rng(40 + L).uniform(0.5, 3.5, 96)
+    llr = sg.gen_llr(9090 + L, 96, ebn0_db=ebn0)
+    bits, st, soft = oracle.decode(llr, L=L, soft=True)
+    try:
+        for backend in (0, 1):
+            decoder.set_params(backend=backend, L=L)
+            r = decoder.decode(llr, soft=True)
+            assert (r.bits == bits).all(), backend
+            assert (r.stats == st).all(), backend
+            assert soft_close(r.soft, soft), soft_err(r.soft, soft)
+    finally:
+        decoder.set_params(L=3)
+
+
 @pytest.mark.parametrize("shape", [
     {"wave_slots": 1, "lane_batch": 8},  # what the autotuner picks on the uneven mix
     {"wave_slots": 3, "lane_batch": 16, "step_len": 256},
