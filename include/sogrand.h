@@ -133,6 +133,21 @@ int sogrand_decode_cubic_noalloc(sogrand_ctx *ctx, const double *llr, size_t n_c
                                  uint8_t *info_bits, double *soft_out, sogrand_cw_stats *stats,
                                  void *hip_stream);
 
+/* Decode n_cw squares of the (16,8)^2 product code: 256 f64 LLRs per square
+ * (index a0 + 16*a1, the layout SOGRAND_C/channel_sim.c writes for dimension
+ * 2), 8 info bytes out per square (info bits b = a0 + 8*a1, MSB first, as
+ * cubic_decoder_final.c:267-280 packs the cube's), optional final L_APP (256
+ * f64 per square) and statistics, with the context's parameters. The square
+ * decode is the reference's per-codeword loop (cubic_decoder_final.c:119-289)
+ * over two axes: phases alternate rows and columns, half an iteration each, and
+ * stop when every row and column of the hard decision is a component codeword.
+ * No counterpart in the reference: its channel simulator accepts the square
+ * code (SOGRAND_C/channel_sim.c:26-29) but it ships no square decoder. Host or
+ * device pointers; synchronous; the backend parameter does not apply (one
+ * kernel: sixteen squares per workgroup). */
+int sogrand_decode_square(sogrand_ctx *ctx, const double *llr, size_t n_cw, uint8_t *info_bits,
+                          double *soft_out, sogrand_cw_stats *stats, void *hip_stream);
+
 /* Component SISO decode of n_lines independent 16-value lines on the GPU:
  * L_APP, L_E (16 f64 per line) and N_guess per line. Replaces SOGRAND_bitSO
  * (cubic_decoder_final.c:521) with L, Tmax, thres from the context.

@@ -101,6 +101,12 @@ class Oracle:
                                              C.c_double, C.c_double, C.c_void_p, C.c_void_p,
                                              C.c_void_p, C.c_void_p, C.c_int]
         lib.og_decode_cubes_cond.restype = C.c_int
+        lib.og_square_is_codeword.argtypes = [_dp]
+        lib.og_square_is_codeword.restype = C.c_int
+        lib.og_decode_squares.argtypes = [_dp, C.c_size_t, C.c_int, C.c_int, C.c_uint64,
+                                          C.c_double, C.c_double, C.c_void_p, C.c_void_p,
+                                          C.c_void_p, C.c_int]
+        lib.og_decode_squares.restype = C.c_int
         lib.og_set_epsilon.argtypes = [C.c_double]
         lib.og_set_epsilon.restype = None
         lib.og_set_variant.argtypes = [C.c_int]
@@ -154,6 +160,22 @@ class Oracle:
 
     def is_codeword(self, cube) -> bool:
         return bool(self.lib.og_cube_is_codeword(np.ascontiguousarray(cube, dtype=np.float64)))
+
+    def is_square_codeword(self, square) -> bool:
+        return bool(self.lib.og_square_is_codeword(np.ascontiguousarray(square, dtype=np.float64)))
+
+    def decode_squares(self, llr, L=3, Imax=30, Tmax=UINT64_MAX, p_et=1e-5, alpha=0.7,
+                       soft=False, threads=0):
+        """Decode squares of the (16,8)^2 code -> (bits (n, 8), stats, soft or None)."""
+        llr = np.ascontiguousarray(llr, dtype=np.float64).reshape(-1, 256)
+        n = llr.shape[0]
+        bits = np.zeros((n, 8), np.uint8)
+        stats = np.zeros(n, CW_STATS_DTYPE)
+        sarr = np.zeros((n, 256)) if soft else None
+        self.lib.og_decode_squares(llr, n, L, Imax, Tmax, p_et, alpha, bits.ctypes.data,
+                                   sarr.ctypes.data if soft else None, stats.ctypes.data,
+                                   threads)
+        return bits, stats, sarr
 
     def decode(self, llr, L=3, Imax=30, Tmax=UINT64_MAX, p_et=1e-5, alpha=0.7, soft=False,
                threads=0, cond=False):

@@ -449,24 +449,37 @@ static int og_line_syndrome(unsigned word)
     return s;
 }
 
-int og_cube_is_codeword(const double *L_APP)
+/* Line geometry of the product code of dimension dim (2 or 3): positions
+ * a0 + 16 a1 (+ 256 a2); line (x, y) of axis ax covers base + j stride[ax],
+ * base = x o1[ax] + y o2[ax], with 16^(dim-1) lines per axis (y = 0 for dim 2).
+ * The square code's axes are the cube's first two. */
+static const int og_stride[3] = {1, 16, 256};
+static const int og_o1[2][3] = {{16, 1, 0}, {16, 1, 1}};
+static const int og_o2[3] = {256, 256, 16};
+
+static int og_is_product_codeword(const double *L_APP, int dim)
 {
     og_init();
-    static const int stride[3] = {1, 16, 256}, o1[3] = {16, 1, 1}, o2[3] = {256, 256, 16};
-    for (int ax = 0; ax < 3; ax++)
-        for (int y = 0; y < 16; y++)
+    const int *o1 = og_o1[dim - 2], ny = dim == 3 ? 16 : 1;
+    for (int ax = 0; ax < dim; ax++)
+        for (int y = 0; y < ny; y++)
             for (int x = 0; x < 16; x++) {
-                int base = x * o1[ax] + y * o2[ax];
+                int base = x * o1[ax] + y * og_o2[ax];
                 unsigned w = 0;
                 for (int j = 0; j < 16; j++)
-                    if (!(L_APP[base + j * stride[ax]] > 0.0)) w |= 1u << j;
+                    if (!(L_APP[base + j * og_stride[ax]] > 0.0)) w |= 1u << j;
                 if (og_line_syndrome(w)) return 0;
             }
     return 1;
 }
 
+int og_cube_is_codeword(const double *L_APP) { return og_is_product_codeword(L_APP, 3); }
+
+int og_square_is_codeword(const double *L_APP) { return og_is_product_codeword(L_APP, 2); }
+
 typedef struct {
     const double *llr;
+    int dim; /* 2: (16,8)^2 square code, 3: (16,8)^3 cubic code */
     size_t n_cw;
     int L, Imax;
     uint64_t Tmax;
@@ -492,27 +505,33 @@ static double og_chain_cond(const og_line_trace *tr)
     return ((double)tr->T + 64.0) * 8.8817841970012523e-16 / fabs(P); /* 2^-50 */
 }
 
-static int og_decode_one(const double *Lch, int L, int Imax, uint64_t Tmax, double thres,
-                         double alpha, uint8_t *bits, double *soft, double *cond,
+/* One codeword of the product code of dimension dim: the reference's
+ * per-codeword loop (final.c:119-289) with one phase per axis; the square code
+ * runs the same loop over its two axes (the reference ships no square decoder:
+ * its channel simulator accepts dimension 2, channel_sim.c:26-29). */
+static int og_decode_one(int dim, const double *Lch, int L, int Imax, uint64_t Tmax,
+                         double thres, double alpha, uint8_t *bits, double *soft, double *cond,
                          og_cw_stats *st)
 {
-    static const int stride[3] = {1, 16, 256}, o1[3] = {16, 1, 1}, o2[3] = {256, 256, 16};
-    double *LE = (double *)calloc(OG_CUBE, sizeof(double));
-    double *LA = (double *)malloc(OG_CUBE * sizeof(double));
-    double *in = (double *)malloc(OG_CUBE * sizeof(double));
+    const int npos = dim == 3 ? OG_CUBE : 256, ny = dim == 3 ? 16 : 1;
+    const int *o1 = og_o1[dim - 2];
+    const int *stride = og_stride, *o2 = og_o2;
+    double *LE = (double *)calloc((size_t)npos, sizeof(double));
+    double *LA = (double *)malloc((size_t)npos * sizeof(double));
+    double *in = (double *)malloc((size_t)npos * sizeof(double));
     int rc = 0;
     uint32_t halves = 0, conv = 0;
     uint64_t ng = 0, ngp = 0;
-    memcpy(LA, Lch, OG_CUBE * sizeof(double));
+    memcpy(LA, Lch, (size_t)npos * sizeof(double));
     for (int it = 1; it <= Imax && !conv; it++) {
-        for (int ax = 0; ax < 3; ax++) {
+        for (int ax = 0; ax < dim; ax++) {
             halves++; /* n_iter += 0.5 before the phase (:145,:184,:223) */
-            for (int i = 0; i < OG_CUBE; i++) {
+            for (int i = 0; i < npos; i++) {
                 double t = alpha * LE[i]; /* L_A, then input, two roundings (:148-155) */
                 in[i] = Lch[i] + t;
             }
             uint32_t gmax = 0;
-            for (int y = 0; y < 16; y++)
+            for (int y = 0; y < ny; y++)
                 for (int x = 0; x < 16; x++) {
                     int base = x * o1[ax] + y * o2[ax];
                     double v[16], la[16], le[16];
@@ -529,20 +548,22 @@ static int og_decode_one(const double *Lch, int L, int Imax, uint64_t Tmax, doub
                     }
                 }
             ngp += gmax;
-            if (og_cube_is_codeword(LA)) {
+            if (og_is_product_codeword(LA, dim)) {
                 conv = 1;
                 break;
             }
         }
     }
     if (bits) {
-        memset(bits, 0, 64);
-        for (int b = 0; b < OG_INFO; b++) {
+        /* info bit b = a0 + 8 a1 (+ 64 a2), MSB first (:267-280) */
+        const int ninfo = dim == 3 ? OG_INFO : 64;
+        memset(bits, 0, (size_t)ninfo / 8);
+        for (int b = 0; b < ninfo; b++) {
             int a0 = b & 7, a1 = (b >> 3) & 7, a2 = b >> 6;
             if (!(LA[a0 + 16 * a1 + 256 * a2] > 0.0)) bits[b >> 3] |= (uint8_t)(0x80u >> (b & 7));
         }
     }
-    if (soft) memcpy(soft, LA, OG_CUBE * sizeof(double));
+    if (soft) memcpy(soft, LA, (size_t)npos * sizeof(double));
     if (st) {
         st->half_iters = halves;
         st->converged = conv;
@@ -563,10 +584,11 @@ static void *og_worker(void *arg)
         size_t c = J->next++;
         pthread_mutex_unlock(&J->mu);
         if (c >= J->n_cw) break;
-        int rc = og_decode_one(J->llr + c * OG_CUBE, J->L, J->Imax, J->Tmax, J->thres, J->alpha,
-                               J->bits ? J->bits + c * 64 : NULL,
-                               J->soft ? J->soft + c * OG_CUBE : NULL,
-                               J->cond ? J->cond + c * OG_CUBE : NULL,
+        const size_t npos = J->dim == 3 ? OG_CUBE : 256, nbytes = J->dim == 3 ? 64 : 8;
+        int rc = og_decode_one(J->dim, J->llr + c * npos, J->L, J->Imax, J->Tmax, J->thres,
+                               J->alpha, J->bits ? J->bits + c * nbytes : NULL,
+                               J->soft ? J->soft + c * npos : NULL,
+                               J->cond ? J->cond + c * npos : NULL,
                                J->stats ? J->stats + c : NULL);
         if (rc) {
             pthread_mutex_lock(&J->mu);
@@ -584,13 +606,14 @@ int og_decode_cubes(const double *llr, size_t n_cw, int L, int Imax, uint64_t Tm
                                 threads);
 }
 
-int og_decode_cubes_cond(const double *llr, size_t n_cw, int L, int Imax, uint64_t Tmax,
-                         double p_et, double alpha, uint8_t *bits, double *soft, double *cond,
-                         og_cw_stats *stats, int threads)
+static int og_decode_product(int dim, const double *llr, size_t n_cw, int L, int Imax,
+                             uint64_t Tmax, double p_et, double alpha, uint8_t *bits,
+                             double *soft, double *cond, og_cw_stats *stats, int threads)
 {
     og_init();
     og_job J;
     J.llr = llr;
+    J.dim = dim;
     J.n_cw = n_cw;
     J.L = L;
     J.Imax = Imax;
@@ -614,4 +637,20 @@ int og_decode_cubes_cond(const double *llr, size_t n_cw, int L, int Imax, uint64
     free(th);
     pthread_mutex_destroy(&J.mu);
     return J.bad ? -1 : 0;
+}
+
+int og_decode_cubes_cond(const double *llr, size_t n_cw, int L, int Imax, uint64_t Tmax,
+                         double p_et, double alpha, uint8_t *bits, double *soft, double *cond,
+                         og_cw_stats *stats, int threads)
+{
+    return og_decode_product(3, llr, n_cw, L, Imax, Tmax, p_et, alpha, bits, soft, cond, stats,
+                             threads);
+}
+
+int og_decode_squares(const double *llr, size_t n_cw, int L, int Imax, uint64_t Tmax,
+                      double p_et, double alpha, uint8_t *bits, double *soft, og_cw_stats *stats,
+                      int threads)
+{
+    return og_decode_product(2, llr, n_cw, L, Imax, Tmax, p_et, alpha, bits, soft, NULL, stats,
+                             threads);
 }

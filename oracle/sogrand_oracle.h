@@ -78,6 +78,9 @@ int og_siso(const double *llr, int L, uint64_t Tmax, double thres, double *L_APP
 /* 1 if the hard decision of the cube (bit = !(x > 0)) is a product codeword. */
 int og_cube_is_codeword(const double *L_APP);
 
+/* The same for a square of the (16,8)^2 code (256 f64, index a0 + 16*a1). */
+int og_square_is_codeword(const double *L_APP);
+
 /* Decode n_cw cubes (4096 f64 each, index a0 + 16*a1 + 256*a2).
  * bits: 64 B per codeword, MSB-first packed info bits b = a0 + 8*a1 + 64*a2.
  * soft (optional): final L_APP, 4096 f64 per codeword. stats (optional).
@@ -92,6 +95,18 @@ int og_decode_cubes(const double *llr, size_t n_cw, int L, int Imax, uint64_t Tm
 int og_decode_cubes_cond(const double *llr, size_t n_cw, int L, int Imax, uint64_t Tmax,
                          double p_et, double alpha, uint8_t *bits, double *soft, double *cond,
                          og_cw_stats *stats, int threads);
+
+/* Decode n_cw squares of the (16,8)^2 product code (256 f64 each, index
+ * a0 + 16*a1) with the same per-codeword loop over two axes: phases alternate
+ * rows (axis 0) and columns (axis 1), n_iter counts half an iteration per
+ * phase, early termination when every row and column is a component codeword.
+ * bits: 8 B per codeword, MSB-first info bits b = a0 + 8*a1. soft (optional):
+ * final L_APP, 256 f64 per codeword. stats (optional). Returns 0. The
+ * reference ships no square decoder (its channel simulator accepts dimension
+ * 2, channel_sim.c:26-29); this is the cubic loop restricted to two axes. */
+int og_decode_squares(const double *llr, size_t n_cw, int L, int Imax, uint64_t Tmax,
+                      double p_et, double alpha, uint8_t *bits, double *soft, og_cw_stats *stats,
+                      int threads);
 
 #ifdef __cplusplus
 }

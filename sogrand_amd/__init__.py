@@ -12,6 +12,7 @@ Mirrors the reference decoder's interface:
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 from dataclasses import dataclass
 
@@ -46,6 +47,7 @@ CW_STATS_DTYPE = np.dtype(
 EXPORTED = [
     "sogrand_params_default", "sogrand_strerror", "sogrand_ctx_create", "sogrand_ctx_set_params",
     "sogrand_ctx_destroy", "sogrand_decode_cubic", "sogrand_decode_cubic_noalloc",
+    "sogrand_decode_square",
     "sogrand_bitso_lines", "sogrand_last_kernel_ms",
     "sogrand_graph_replays",
     "sogrand_sigma_for_ebn0", "sogrand_gen_info_bits", "sogrand_encode_cubes", "sogrand_gen_llr",
@@ -129,6 +131,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.sogrand_ctx_destroy.restype = None
     lib.sogrand_decode_cubic.argtypes = [vp, vp, C.c_size_t, vp, vp, vp, vp]
     lib.sogrand_bitso_lines.argtypes = [vp, vp, C.c_size_t, vp, vp, vp, vp]
+    lib.sogrand_decode_square.argtypes = [vp, vp, C.c_size_t, vp, vp, vp, vp]
     lib.sogrand_decode_cubic_noalloc.argtypes = [vp, vp, C.c_size_t, vp, vp, vp, vp]
     lib.sogrand_last_kernel_ms.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_uint32)]
     lib.sogrand_graph_replays.argtypes = [vp, C.POINTER(C.c_uint32)]
@@ -251,6 +254,26 @@ class Decoder:
         rc = self._lib.sogrand_decode_cubic(self._h, _ptr(llr), n, _ptr(bits), _ptr(sf), _ptr(st),
                                             stream)
         _check(rc, "decode_cubic")
+        return DecodeResult(bits, st, sf)
+
+    def decode_square(self, llr, soft: bool = False, stats: bool = True, out_bits=None,
+                      out_stats=None, out_soft=None, stream=None) -> DecodeResult:
+        """Decode squares of the (16,8)^2 code (sogrand_decode_square). llr: (n, 256)
+        float64, host (numpy) or device (torch) memory; bits (n, 8)."""
+        if isinstance(llr, np.ndarray):
+            llr = np.ascontiguousarray(llr, dtype=np.float64).reshape(-1, 256)
+            n = llr.shape[0]
+            bits = out_bits if out_bits is not None else np.zeros((n, 8), np.uint8)
+            st = out_stats if out_stats is not None else (np.zeros(n, CW_STATS_DTYPE) if stats else None)
+            sf = out_soft if out_soft is not None else (np.zeros((n, 256)) if soft else None)
+        else:
+            n = llr.numel() // 256
+            bits, st, sf = out_bits, out_stats, out_soft
+            if bits is None:
+                raise ValueError("device decode needs out_bits (and optional out_stats/out_soft)")
+        rc = self._lib.sogrand_decode_square(self._h, _ptr(llr), n, _ptr(bits), _ptr(sf),
+                                             _ptr(st), stream)
+        _check(rc, "decode_square")
         return DecodeResult(bits, st, sf)
 
     def decode_noalloc(self, llr, out_bits, out_stats=None, out_soft=None, stream=None) -> None:
@@ -380,8 +403,16 @@ def schedule_table(ic: int) -> np.ndarray:
     return out
 
 
-def sigma_for_ebn0(ebn0_db: float) -> float:
-    return load_library().sogrand_sigma_for_ebn0(ebn0_db)
+def sigma_for_ebn0(ebn0_db: float, dim: int = 3) -> float:
+    """Noise sigma of SOGRAND_C/channel_sim.c:32-36 for the cubic (dim 3) or
+    square (dim 2) code: rate (8/16)^dim, Es/N0 = Eb/N0 + 10 log10(2 R)."""
+    if dim == 3:
+        return load_library().sogrand_sigma_for_ebn0(ebn0_db)
+    if dim != 2:
+        raise ValueError("dim must be 2 or 3")
+    r = (8.0 / 16.0) ** 2
+    esn0 = ebn0_db + 10.0 * math.log10(2.0 * r)
+    return 1.0 / math.sqrt(10.0 ** (esn0 / 10.0))
 
 
 def gen_info_bits(seed: int, n: int, cw_offset: int = 0) -> np.ndarray:
@@ -396,6 +427,33 @@ def encode_cubes(info_bits: np.ndarray) -> np.ndarray:
     _check(load_library().sogrand_encode_cubes(info_bits.ctypes.data, info_bits.shape[0],
                                                out.ctypes.data), "encode")
     return out
+
+
+def encode_squares(info_bits: np.ndarray) -> np.ndarray:
+    """Codewords of the (16,8)^2 square code: (n, 8) info bytes (bit b = a0 + 8 a1,
+    MSB first) -> (n, 256) code bits by position a0 + 16 a1. A square codeword is
+    the plane a2 = 0 of the cubic codeword whose other info planes are zero (the
+    product code's encoding leaves an information plane a square codeword)."""
+    info_bits = np.ascontiguousarray(info_bits, dtype=np.uint8).reshape(-1, 8)
+    cube_info = np.zeros((info_bits.shape[0], 64), np.uint8)
+    cube_info[:, :8] = info_bits
+    code = np.unpackbits(encode_cubes(cube_info), axis=1)  # bit index a0 + 16 a1 + 256 a2
+    return code[:, :256].copy()
+
+
+def gen_llr_square(seed: int, n: int, ebn0_db=None, sigma=None):
+    """(info (n, 8), LLRs (n, 256)) of n random squares through BPSK and AWGN
+    (numpy's generator seeded with seed; LLR = 2 y / sigma^2 as in
+    channel_sim.c). Give ebn0_db (scalar or per-square array) or sigma."""
+    rng = np.random.default_rng(seed)
+    info = rng.integers(0, 256, (n, 8), dtype=np.uint8)
+    if sigma is None:
+        e = np.broadcast_to(np.asarray(ebn0_db, dtype=np.float64), (n,))
+        sigma = np.array([sigma_for_ebn0(float(x), dim=2) for x in e])
+    sig = np.broadcast_to(np.asarray(sigma, dtype=np.float64), (n,))[:, None]
+    x = 1.0 - 2.0 * encode_squares(info).astype(np.float64)
+    y = x + sig * rng.standard_normal((n, 256))
+    return info, 2.0 * y / (sig * sig)
 
 
 def gen_llr(seed: int, n: int, ebn0_db=None, sigma=None, cw_offset: int = 0) -> np.ndarray:

@@ -17,6 +17,9 @@
 //                       {u32 half_iters, u32 converged, u64 NG, u64 NG_p}
 //   --soft-out FILE     final L_APP, 4096 float64 per codeword
 //   --device N          HIP device ordinal (default 0)
+//   --dim D             3 (default: the cubic code) | 2 (the (16,8)^2 square code:
+//                       256 float64 LLRs in and 8 info bytes out per codeword, the
+//                       dimension SOGRAND_C/channel_sim.c also accepts)
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -41,7 +44,7 @@ int main(int argc, char *argv[])
     sogrand_params p;
     sogrand_params_default(&p);
     const char *stats_path = nullptr, *soft_path = nullptr;
-    int device = 0;
+    int device = 0, dim = 3;
     for (int i = 3; i < argc; ++i) {
         std::string a = argv[i];
         auto need = [&](void) -> const char * {
@@ -76,6 +79,9 @@ int main(int argc, char *argv[])
             soft_path = need();
         } else if (a == "--device") {
             device = std::atoi(need());
+        } else if (a == "--dim") {
+            dim = std::atoi(need());
+            if (dim != 2 && dim != 3) return usage(argv[0]);
         } else {
             return usage(argv[0]);
         }
@@ -89,8 +95,10 @@ int main(int argc, char *argv[])
     std::fseek(fin, 0, SEEK_END);
     const long long file_size = std::ftell(fin);
     std::rewind(fin);
+    // positions and info bytes of one codeword of the code
+    const size_t npos = dim == 3 ? SOGRAND_CUBE : 256, ninfo = dim == 3 ? SOGRAND_INFO_BYTES : 8;
     // a trailing partial codeword is ignored (cubic_decoder1.cu:997, final.c:119)
-    const size_t total = (size_t)(file_size / (long long)(SOGRAND_CUBE * sizeof(double)));
+    const size_t total = (size_t)(file_size / (long long)(npos * sizeof(double)));
     if (total == 0) {
         std::fprintf(stderr, "Error: No valid blocks found in input file\n");
         std::fclose(fin);
@@ -116,17 +124,16 @@ int main(int argc, char *argv[])
         return 1;
     }
     if (p.variant == SOGRAND_VARIANT_LEGACY)
-        std::printf("Decoding %s to %s using cubic code (n=%d, k=%d)...\n", in_path, out_path,
-                    SOGRAND_N, SOGRAND_K);
+        std::printf("Decoding %s to %s using %s code (n=%d, k=%d)...\n", in_path, out_path,
+                    dim == 3 ? "cubic" : "square", SOGRAND_N, SOGRAND_K);
     // chunks of the file alternate between two buffers: the next one is read by
     // a second thread while the current one decodes
     const size_t chunk = 65536;
     std::vector<double> llr[2];
     bool read_ok[2] = {false, false};
     auto read_chunk = [&](int b, size_t n) {
-        llr[b].resize(n * SOGRAND_CUBE);
-        read_ok[b] = std::fread(llr[b].data(), sizeof(double), n * SOGRAND_CUBE, fin) ==
-                     n * SOGRAND_CUBE;
+        llr[b].resize(n * npos);
+        read_ok[b] = std::fread(llr[b].data(), sizeof(double), n * npos, fin) == n * npos;
     };
     std::vector<uint8_t> bits;
     std::vector<sogrand_cw_stats> st;
@@ -142,16 +149,17 @@ int main(int argc, char *argv[])
             const size_t left = total - off - chunk;
             reader = std::thread(read_chunk, cur ^ 1, left < chunk ? left : chunk);
         }
-        bits.resize(n * SOGRAND_INFO_BYTES);
+        bits.resize(n * ninfo);
         st.resize(n);
-        if (fsoft) soft.resize(n * SOGRAND_CUBE);
+        if (fsoft) soft.resize(n * npos);
         if (!read_ok[cur]) {
             if (reader.joinable()) reader.join();
             std::fprintf(stderr, "Error: Failed to read entire file\n");
             return 1;
         }
-        rc = sogrand_decode_cubic(ctx, llr[cur].data(), n, bits.data(),
-                                  fsoft ? soft.data() : nullptr, st.data(), nullptr);
+        rc = (dim == 3 ? sogrand_decode_cubic : sogrand_decode_square)(
+            ctx, llr[cur].data(), n, bits.data(), fsoft ? soft.data() : nullptr, st.data(),
+            nullptr);
         if (reader.joinable()) reader.join();
         if (rc) {
             std::fprintf(stderr, "Error: %s\n", sogrand_strerror(rc));
@@ -180,7 +188,8 @@ int main(int argc, char *argv[])
     std::printf("Decoding complete. %zu block(s) decoded.\n", total);
     std::printf("Average iterations per block: %.2f\n", (double)total_half / 2.0 / blocks);
     std::printf("Average NG per block: %.2f\n", (double)total_ng / blocks);
-    std::printf("Average NG per info bit: %.2f\n", (double)total_ng / (blocks * 512.0));
+    std::printf("Average NG per info bit: %.2f\n",
+                (double)total_ng / (blocks * (dim == 3 ? 512.0 : 64.0)));
     std::printf("Average NG_p per block: %.2f\n", (double)total_ngp / blocks);
     std::printf("Total GPU time: %.2f seconds\n", gpu_s);
     std::printf("Throughput: %.2f blocks/sec\n", gpu_s > 0 ? blocks / gpu_s : 0.0);

@@ -563,6 +563,76 @@ int sogrand_decode_cubic(sogrand_ctx *ctx, const double *llr, size_t n_cw, uint8
     return rc;
 }
 
+int sogrand_decode_square(sogrand_ctx *ctx, const double *llr, size_t n_cw, uint8_t *info_bits,
+                          double *soft_out, sogrand_cw_stats *stats, void *hip_stream)
+{
+    if (!ctx || (!llr && n_cw) || (!info_bits && n_cw)) return SOGRAND_EINVAL;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    ctx->last_ms = 0.0;
+    ctx->last_launches = 0;
+    if (n_cw == 0) return SOGRAND_OK;
+    if (hipSetDevice(ctx->device) != hipSuccess) return SOGRAND_EHIP;
+    hipStream_t st = hip_stream ? (hipStream_t)hip_stream : ctx->own_stream;
+    constexpr size_t kPos = 256, kInfo = 8;
+    // host buffers are staged through the context's device buffers whole (a
+    // square is 2 KB of LLRs: a million squares stage 2 GB)
+    const double *d_llr = llr;
+    if (!is_device_ptr(llr)) {
+        if (ctx->llr_stage.reserve(n_cw * kPos) != hipSuccess) return SOGRAND_ENOMEM;
+        if (hipMemcpyAsync(ctx->llr_stage.p, llr, n_cw * kPos * sizeof(double),
+                           hipMemcpyHostToDevice, st) != hipSuccess)
+            return SOGRAND_EHIP;
+        d_llr = ctx->llr_stage.p;
+    }
+    const bool dev_info = is_device_ptr(info_bits);
+    uint8_t *d_info = info_bits;
+    if (!dev_info) {
+        if (ctx->info_stage.reserve(n_cw * kInfo) != hipSuccess) return SOGRAND_ENOMEM;
+        d_info = ctx->info_stage.p;
+    }
+    const bool dev_soft = soft_out && is_device_ptr(soft_out);
+    double *d_soft = soft_out;
+    if (soft_out && !dev_soft) {
+        if (ctx->soft_stage.reserve(n_cw * kPos) != hipSuccess) return SOGRAND_ENOMEM;
+        d_soft = ctx->soft_stage.p;
+    }
+    const bool dev_stats = stats && is_device_ptr(stats);
+    uint32_t *d_stats = (uint32_t *)stats;
+    if (stats && !dev_stats) {
+        if (ctx->stats_stage.reserve(n_cw * 6) != hipSuccess) return SOGRAND_ENOMEM;
+        d_stats = ctx->stats_stage.p;
+    }
+    sg::CubeJob job{};
+    job.llr = d_llr;
+    job.info = d_info;
+    job.soft = d_soft;
+    job.stats = d_stats;
+    job.n_cw = n_cw;
+    sg::fill_job(job, ctx, ctx->p);
+    const size_t blocks = sg::square_blocks(n_cw);
+    if (ctx->le_scratch.reserve(blocks * sg::kCube) != hipSuccess) return SOGRAND_ENOMEM;
+    job.le_scratch = ctx->le_scratch.p;
+    (void)hipEventRecord(ctx->ev0, st);
+    if (sg::launch_square(job, (int)blocks, st) != hipSuccess) return SOGRAND_EHIP;
+    (void)hipEventRecord(ctx->ev1, st);
+    if (hipEventSynchronize(ctx->ev1) != hipSuccess) return SOGRAND_EHIP;
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    ctx->last_ms = ms;
+    ctx->last_launches = 1;
+    if (!dev_info && hipMemcpy(info_bits, d_info, n_cw * kInfo, hipMemcpyDeviceToHost) != hipSuccess)
+        return SOGRAND_EHIP;
+    if (soft_out && !dev_soft &&
+        hipMemcpy(soft_out, d_soft, n_cw * kPos * sizeof(double), hipMemcpyDeviceToHost) !=
+            hipSuccess)
+        return SOGRAND_EHIP;
+    if (stats && !dev_stats &&
+        hipMemcpy(stats, d_stats, n_cw * sizeof(sogrand_cw_stats), hipMemcpyDeviceToHost) !=
+            hipSuccess)
+        return SOGRAND_EHIP;
+    return SOGRAND_OK;
+}
+
 int sogrand_bitso_lines(sogrand_ctx *ctx, const double *llr, size_t n_lines, double *L_APP,
                         double *L_E, uint32_t *N_guess, void *hip_stream)
 {

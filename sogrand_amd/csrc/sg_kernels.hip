@@ -6,6 +6,9 @@
 //                  reference GPU kernel highly_optimized_sogrand_kernel
 //                  (SOGRAND_CUDA/cubic_decoder1.cu:635-841) with the CPU
 //                  decoder's semantics (SOGRAND_C/cubic_decoder_final.c:119-289).
+// square_kernel  : the (16,8)^2 square code: sixteen squares per workgroup, a
+//                  16-lane group per square, one lane per line, phases
+//                  alternating rows and columns.
 // lines_kernel   : SOGRAND_bitSO (cubic_decoder_final.c:521) on independent lines.
 // gen_kernel     : seeded info bits -> product encoding -> BPSK/AWGN -> LLR.
 #include <hip/hip_runtime.h>
@@ -184,6 +187,127 @@ __global__ __launch_bounds__(kPlainThreads) void plain_kernel(CubeJob job)
     }
 }
 
+// The (16,8)^2 square code (256 positions a0 + 16 a1, info bits b = a0 + 8 a1):
+// the reference's per-codeword loop (cubic_decoder_final.c:119-289) over two
+// axes. The reference ships no square decoder; its channel simulator accepts
+// the square code (SOGRAND_C/channel_sim.c:26-29). Phases alternate rows
+// (axis 0: line i = positions 16 i + j) and columns (axis 1: i + 16 j), half an
+// iteration each, with early termination when every row and column of the
+// hard decision is a component codeword. A workgroup takes sixteen squares at
+// a time, the 16-lane group 16 g .. 16 g + 15 square g (groups never straddle
+// a wave, so a group's lines exchange hard decisions by ballots, and groups run
+// independently: no workgroup barrier after the table load). L_E sits in the
+// workgroup's 4096-double scratch slot, 256 doubles per square; a workgroup
+// fence orders a phase's stores before the next phase's loads, which other
+// lanes of the same wave make.
+constexpr int kSquareThreads = 256;
+constexpr int kSquarePos = 256;
+constexpr int kSquareInfoBytes = 8;
+
+// For the 16-lane group of the calling lane: bit j of the result, in lane
+// 16 g + i, is bit i of lane 16 g + j's word w (the hard decisions of the
+// group's lines transposed: rows <-> columns).
+__device__ __forceinline__ uint32_t group16_transpose(uint32_t w, int i, int gshift)
+{
+    uint32_t out = 0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const uint32_t col = (uint32_t)(__ballot((w >> k) & 1u) >> gshift) & 0xffffu;
+        out = k == i ? col : out;
+    }
+    return out;
+}
+
+template <bool FINITE_T, class P>
+__global__ __launch_bounds__(kSquareThreads) void square_kernel(CubeJob job)
+{
+    using Rec = typename P::Rec;
+    __shared__ Rec s_delta[17 * kSquareThreads];
+    __shared__ uint16_t s_hp[17 * kSquareThreads];
+    __shared__ ExpEntry s_exp[kLdsTableEntries];
+    const int t = threadIdx.x, g = t >> 4, i = t & 15, gshift = 16 * (g & 3);
+    load_exp_table(s_exp, t, job.eps, job.thres);
+    __syncthreads(); // the only workgroup barrier: groups run independently from here
+    const LineLdsT<Rec, kSquareThreads> S(s_delta, s_hp, t);
+    const DecodeCfg cfg{job.tmax, job.thres, job.sched, job.L, s_exp, kLogTable, job.eps, job.eps_log};
+    double *le = job.le_scratch + (size_t)blockIdx.x * kCube + (size_t)kSquarePos * g;
+
+    for (size_t sq = (size_t)blockIdx.x * 16 + g; sq < job.n_cw; sq += (size_t)gridDim.x * 16) {
+        const double *lch = job.llr + sq * kSquarePos;
+        double *soft = job.soft ? job.soft + sq * kSquarePos : nullptr;
+        uint32_t half = 0, conv = 0, w = 0, tw = 0;
+        uint64_t ng = 0, ngp = 0;
+        for (uint32_t it = 1; it <= job.Imax && !conv; ++it) {
+            for (int ax = 0; ax < 2; ++ax) {
+                ++half; // n_iter += 0.5 before each phase (:145, :184)
+                CubeSrc src{lch, le, job.alpha, ax == 0 ? 16 * i : i, ax == 0 ? 1 : 16,
+                            it == 1 && ax == 0};
+                uint32_t hd, s0;
+                double pm_hd;
+                int32_t ic;
+                LineList ls;
+                line_setup_p<P>(src, S, hd, s0, pm_hd, ic, cfg);
+                line_search<FINITE_T, P>(S, hd, s0, pm_hd, ic, cfg, ls);
+                const double pnotin = fmax(ls.pnl, cfg.eps);
+                ListEntries ent;
+                list_entries<P>(ls, S, hd, pm_hd, ic, cfg, ent);
+                w = 0;
+#pragma unroll 1
+                for (int j = 0; j < 16; ++j) {
+                    const int p = src.base + j * src.stride;
+                    const double xin = src(j);
+                    double la, lenew;
+                    if (ls.nl == 0) { // empty list: L_APP = llr, L_E = 0 (:537-539)
+                        la = xin;
+                        lenew = 0.0;
+                    } else {
+                        la = soft_app(ent, ls.nl, pnotin, j, xin, cfg);
+                        lenew = la - xin;
+                    }
+                    // within a phase each position belongs to one line, and a
+                    // lane reads a position (src) before it overwrites it
+                    le[p] = lenew;
+                    if (soft) soft[p] = la;
+                    if (!(la > 0.0)) w |= 1u << j;
+                }
+                // the phase's L_E stores before the next phase's loads (other
+                // lanes of the same group read them)
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+                // NG += N_guess; NG_p += the phase's maximum (:162-163, :170)
+                uint32_t sum = ls.T, mx = ls.T;
+#pragma unroll
+                for (int o = 8; o >= 1; o >>= 1) {
+                    sum += (uint32_t)__shfl_xor((int)sum, o, 64);
+                    const uint32_t m2 = (uint32_t)__shfl_xor((int)mx, o, 64);
+                    mx = m2 > mx ? m2 : mx;
+                }
+                ng += sum;
+                ngp += mx;
+                // early termination: every line of both axes is a component
+                // codeword (the re-encode test of :317-391 for two axes)
+                tw = group16_transpose(w, i, gshift);
+                const bool ok = syndrome16(w) == 0 && syndrome16(tw) == 0;
+                if (((__ballot(!ok) >> gshift) & 0xffffull) == 0) {
+                    conv = 1;
+                    w = ax == 0 ? w : tw; // rows: lane i holds row a1 = i
+                    break;
+                }
+                if (ax == 1) w = tw;
+            }
+        }
+        // w holds row a1 = i of the final hard decision (bit a0); info bit
+        // b = a0 + 8 a1, MSB first (:267-280)
+        if (i < kSquareInfoBytes) job.info[sq * kSquareInfoBytes + i] = (uint8_t)rev8(w & 0xffu);
+        if (i == 0 && job.stats) {
+            CwStatsDev *st = reinterpret_cast<CwStatsDev *>(job.stats) + sq;
+            st->half_iters = half;
+            st->converged = conv;
+            st->NG = ng;
+            st->NG_p = ngp;
+        }
+    }
+}
+
 template <bool FINITE_T, class P>
 __global__ __launch_bounds__(256) void lines_kernel(const double *llr, size_t n, double *la,
                                                     double *le, uint32_t *nguess, DecodeCfg cfg)
@@ -313,6 +437,30 @@ hipError_t launch_plain(const CubeJob &job, int grid, hipStream_t st)
         if (finite) SG_PLAIN(true, PrecF64); else SG_PLAIN(false, PrecF64);
     }
 #undef SG_PLAIN
+    return hipGetLastError();
+}
+
+size_t square_blocks(size_t n_sq)
+{
+    const size_t cap = 8192; // 131,072 squares in flight
+    const size_t b = (n_sq + 15) / 16;
+    return b < cap ? b : cap;
+}
+
+hipError_t launch_square(const CubeJob &job, int grid, hipStream_t st)
+{
+    if (job.n_cw == 0) return hipSuccess;
+    const bool finite = tmax_binds(job.tmax);
+    const dim3 g(grid), b(kSquareThreads);
+#define SG_SQUARE(FT, PP) hipLaunchKernelGGL((square_kernel<FT, PP>), g, b, 0, st, job)
+    if (job.arith == kArithF32) {
+        if (finite) SG_SQUARE(true, PrecF32); else SG_SQUARE(false, PrecF32);
+    } else if (job.arith == kArithF64Legacy) {
+        if (finite) SG_SQUARE(true, PrecF64Legacy); else SG_SQUARE(false, PrecF64Legacy);
+    } else {
+        if (finite) SG_SQUARE(true, PrecF64); else SG_SQUARE(false, PrecF64);
+    }
+#undef SG_SQUARE
     return hipGetLastError();
 }
 

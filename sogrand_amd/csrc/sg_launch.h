@@ -43,6 +43,12 @@ inline bool tmax_binds(uint64_t tmax) { return tmax < 65536u; }
 hipError_t launch_plain(const CubeJob &job, int grid, hipStream_t st);
 size_t plain_scratch_slots(size_t n_cw);
 
+// The (16,8)^2 square code: sixteen squares per 256-lane workgroup, 256 LLRs and
+// 8 info bytes per square (llr/info/soft of the job in those units); the job's
+// scratch holds one 4096-double cube per workgroup.
+hipError_t launch_square(const CubeJob &job, int grid, hipStream_t st);
+size_t square_blocks(size_t n_squares);
+
 // Persistent backend: waves pop cubes from a device counter; each wave keeps
 // `slots` cubes in flight and hands lines to lanes as they free up.
 hipError_t launch_persistent(const CubeJob &job, int waves_per_block, int blocks, int slots,
