@@ -17,6 +17,7 @@ Other workloads, each printing its own JSON line:
     --workload mixed-precision  1,000,000 codewords at 2.0 and 2.5 dB: FP32 variant vs FP64
     --workload serving          Poisson frames of U{1..256} codewords at U[1, 3] dB, offered
                                 load 10-120 % of capacity: p50/p99 latency, sustained cw/s
+    --workload square           100,000 squares of the (16,8)^2 code at 1, 2 and 3 dB
 """
 from __future__ import annotations
 
@@ -501,6 +502,43 @@ def workload_variants(args) -> dict:
                                    f"L=3, Imax=30", "n_cw": n}, "rows": rows}
 
 
+def workload_square(args) -> dict:
+    """The (16,8)^2 square code: 100,000 squares per Eb/N0 (1, 2, 3 dB) in device
+    memory, decoded by sogrand_decode_square; throughput and error rates."""
+    import sogrand_amd as sg
+    from sogrand_amd import hip
+
+    n = args.n_cw
+    dec = sg.Decoder(0)
+    rows = []
+    for snr in (1.0, 2.0, 3.0):
+        info, llr = sg.gen_llr_square(args.seed + int(snr * 10), n, ebn0_db=snr)
+        d_llr = hip.DeviceArray.from_host(llr)
+        d_bits = hip.DeviceArray((n, 8), np.uint8)
+        d_st = hip.DeviceArray((n,), sg.CW_STATS_DTYPE)
+        dec.decode_square(d_llr, out_bits=d_bits, out_stats=d_st)  # warm-up
+        hip.synchronize()
+        t0 = time.perf_counter()
+        dec.decode_square(d_llr, out_bits=d_bits, out_stats=d_st)
+        wall = time.perf_counter() - t0
+        kms = dec.last_kernel_ms()[0]
+        bits, st = d_bits.to_host(), d_st.to_host()
+        be, bl = ber_bler(bits, info)
+        rows.append({"ebn0_db": snr, "cw_s": round(n / wall, 1), "kernel_ms": round(kms, 2),
+                     "ber": be / (64.0 * n), "ber_ci95": wilson(be, 64 * n), "bler": bl / n,
+                     "bler_ci95": wilson(bl, n), "mean_half_iters": float(st["half_iters"].mean()),
+                     "converged": float(st["converged"].mean()),
+                     "mean_NG": float(st["NG"].astype(np.float64).mean())})
+        log(f"square {snr} dB: {n / wall:.0f} cw/s, BLER {bl / n:.3e}")
+        for x in (d_llr, d_bits, d_st):
+            x.free()
+    return {"metric": "codewords/s of the (16,8)^2 square code", "unit": "codewords/s",
+            "n_gpus": 1, "dtype": "f64",
+            "data": "synthetic (numpy-seeded info bits, (16,8)^2 encoding, BPSK, AWGN at rate 1/4)",
+            "config": {"workload": f"{n} squares at 1.0, 2.0 and 3.0 dB, L=3, Imax=30",
+                       "n_cw": n}, "rows": rows}
+
+
 # ---------------------------------------------------------------------------
 # HBM counter calibration
 
@@ -682,7 +720,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="headline",
-                    choices=["headline", "snr-sweep", "uneven", "mixed-precision", "serving",
+                    choices=["headline", "snr-sweep", "uneven", "mixed-precision", "serving", "square",
                              "variants", "copy"])
     ap.add_argument("--n-cw", type=int, default=None)
     ap.add_argument("--ebn0", type=float, default=2.0)
@@ -706,7 +744,7 @@ def main() -> int:
     if args.n_cw is None:
         args.n_cw = 100_000
     fn = {"headline": workload_headline, "uneven": workload_uneven,
-          "mixed-precision": workload_mixed, "serving": workload_serving,
+          "mixed-precision": workload_mixed, "serving": workload_serving, "square": workload_square,
           "variants": workload_variants, "snr-sweep": workload_snr_sweep,
           "copy": workload_copy}[args.workload]
     print(json.dumps(fn(args)), flush=True)

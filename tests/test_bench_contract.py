@@ -20,7 +20,8 @@ def test_bench_help_lists_the_workloads():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--help"],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0
-    for w in ("headline", "snr-sweep", "uneven", "mixed-precision", "serving", "variants", "copy"):
+    for w in ("headline", "snr-sweep", "uneven", "mixed-precision", "serving", "variants", "copy",
+              "square"):
         assert w in r.stdout
 
 
