@@ -45,12 +45,6 @@ def within_chain_bound(a, b, kappa) -> np.ndarray:
     return np.abs(a - b) <= SOFT_RTOL * np.maximum(np.abs(b), 1.0) + 4.0 * kappa
 
 
-def frac_within(a, b, tol=SOFT_RTOL) -> float:
-    a = np.asarray(a)
-    b = np.asarray(b)
-    return float(np.mean(np.abs(a - b) <= tol * np.maximum(np.abs(b), 1.0)))
-
-
 def stable_codewords(st_a, st_b, bits_a, bits_b) -> np.ndarray:
     """Codewords on which two valid evaluations of the reference arithmetic agree."""
     same = (bits_a == bits_b).all(axis=1)

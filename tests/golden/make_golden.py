@@ -141,8 +141,8 @@ def run_ref_binary_one(llr: np.ndarray, binary: str = REF_BIN) -> tuple[bytes, f
 
 def main() -> None:
     ap = argparse.ArgumentParser()
-    ap.add_argument("--cubes-per-snr", type=int, default=200)
-    ap.add_argument("--soft-per-snr", type=int, default=4)
+    ap.add_argument("--cubes-per-snr", type=int, default=2000)
+    ap.add_argument("--soft-per-snr", type=int, default=20)
     ap.add_argument("--workers", type=int, default=os.cpu_count() or 8)
     args = ap.parse_args()
     if not build_reference():
@@ -253,10 +253,9 @@ def main() -> None:
             assert hi == half[c] and ng_c == ngc[c] and ngp_c == ngpc[c]
             assert (np.packbits(b) == bits[c]).all()
             soft[c] = la_c
-        soft_f = np.array([ref_fma.decode_cube(llr[c])[0] for c in range(ns)])
         np.savez_compressed(os.path.join(HERE, f"cubes_{snr:.1f}.npz"), bits=bits, half_iters=half,
                             NG=ngc, NG_p=ngpc, soft=soft, bits_fma=bits_f, half_iters_fma=half_f,
-                            NG_fma=ng_f, NG_p_fma=ngp_f, soft_fma=soft_f)
+                            NG_fma=ng_f, NG_p_fma=ngp_f)
         print(f"snr {snr}: {n} cw, mean half {half.mean():.2f} NG {ngc.mean():.0f}", flush=True)
     manifest["cubes"] = seeds
     with open(os.path.join(HERE, "manifest.json"), "w") as f:

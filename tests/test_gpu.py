@@ -19,7 +19,7 @@ import sys
 import numpy as np
 import pytest
 
-from conftest import GOLDEN, ROOT, frac_within, golden, soft_close, soft_err, stable_codewords
+from conftest import GOLDEN, ROOT, golden, soft_close, soft_err, stable_codewords
 
 pytestmark = pytest.mark.gpu
 
@@ -89,8 +89,8 @@ def test_cubes_match_reference_binary(decoder, oracle, backend, snr):
     # final soft outputs: the reference's (recorded for the first codewords)
     ns = g["soft"].shape[0]
     assert soft_close(r.soft[:ns], g["soft"]), soft_err(r.soft[:ns], g["soft"])
-    _say(f"  soft: within 1e-9 everywhere (max {soft_err(r.soft[:ns], g['soft']):.2e}); "
-         f"the reference's FMA build: {frac_within(g['soft_fma'], g['soft']):.3f} within 1e-9")
+    _say(f"  soft of {ns} codewords: within 1e-9 everywhere "
+         f"(max {soft_err(r.soft[:ns], g['soft']):.2e})")
 
 
 def test_lines_with_large_metrics_match_oracle(decoder, oracle):
