@@ -670,17 +670,84 @@ template <int I> __device__ __forceinline__ uint32_t pos_field(uint32_t w)
     return f;
 }
 
+// Path metrics and syndromes of the patterns of the position words w1 and w2
+// (fields 0..5: record rows, 0 = none): pm = PM_HD + the increments in
+// ascending sorted position, as getPM adds them (:765), and s0 XOR the H
+// columns. In double precision all twelve fields' records are read before the
+// sums: the reads do not depend on the sums, so their latencies overlap (3.8 %
+// faster than reading and summing two fields at a time); a lane whose pattern
+// has fewer bits reads the null record (row 0), which leaves both sums
+// unchanged. In single precision, whose rounds are cheaper, a round of two
+// fields per pattern runs only if some lane of the wave needs it (5 % faster
+// there). Without path metrics (the earlier decoder without a budget) only the
+// syndromes are formed.
+template <class P, bool kPmLoop, class LDS>
+__device__ __forceinline__ void pattern_sums(const LDS &S, uint32_t s0, double pm_hd, uint32_t w1,
+                                             uint32_t w2, typename P::Rec &pm1,
+                                             typename P::Rec &pm2, uint32_t &syn1,
+                                             uint32_t &syn2)
+{
+    using Rec = typename P::Rec;
+    const Rec pm_base = (Rec)pm_hd;
+    if constexpr (!P::kF32 && kPmLoop) {
+        const uint32_t r1[6] = {pos_field<0>(w1), pos_field<1>(w1), pos_field<2>(w1),
+                                pos_field<3>(w1), pos_field<4>(w1), pos_field<5>(w1)};
+        const uint32_t r2[6] = {pos_field<0>(w2), pos_field<1>(w2), pos_field<2>(w2),
+                                pos_field<3>(w2), pos_field<4>(w2), pos_field<5>(w2)};
+        Rec d1[6], d2[6];
+        uint32_t h1[6], h2[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            d1[k] = S.dr(r1[k]);
+            d2[k] = S.dr(r2[k]);
+            h1[k] = S.hr(r1[k]);
+            h2[k] = S.hr(r2[k]);
+        }
+        pm1 = pm_base;
+        pm2 = pm_base;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            pm1 = pm1 + d1[k];
+            pm2 = pm2 + d2[k];
+        }
+        syn1 = xor3(xor3(xor3(s0, h1[0], h1[1]), h1[2], h1[3]), h1[4], h1[5]);
+        syn2 = xor3(xor3(xor3(s0, h2[0], h2[1]), h2[2], h2[3]), h2[4], h2[5]);
+    } else {
+        auto round = [&](auto i0) {
+            constexpr int I = decltype(i0)::value;
+            const uint32_t a1 = pos_field<I>(w1), b1 = pos_field<I + 1>(w1);
+            const uint32_t a2 = pos_field<I>(w2), b2 = pos_field<I + 1>(w2);
+            syn1 = xor3(syn1, (uint32_t)S.hr(a1), (uint32_t)S.hr(b1));
+            syn2 = xor3(syn2, (uint32_t)S.hr(a2), (uint32_t)S.hr(b2));
+            if constexpr (kPmLoop) {
+                pm1 = (pm1 + S.dr(a1)) + S.dr(b1);
+                pm2 = (pm2 + S.dr(a2)) + S.dr(b2);
+            }
+        };
+        syn1 = s0;
+        syn2 = s0;
+        pm1 = pm_base;
+        pm2 = pm_base;
+        round(IntC<0>{});
+        if constexpr (P::kF32) {
+            // fields 2.. (and the overflow bit) sit above bit 10, fields 4.. above bit 20
+            if (__ballot(((w1 | w2) >> 11) != 0u) != 0ull) {
+                round(IntC<2>{});
+                if (__ballot(((w1 | w2) >> 21) != 0u) != 0ull) round(IntC<4>{});
+            }
+        } else {
+            round(IntC<2>{});
+            round(IntC<4>{});
+        }
+    }
+}
+
 // Two consecutive guesses of the schedule (:652-744): the patterns of the
 // position words w1 and w2 at schedule positions t and t + 1. Each lane
 // carries two independent path-metric and exp chains; the guesses are still
 // accounted strictly in schedule order (the second is dropped if the first
-// ends the search). The fields of both words are read two at a time in three
-// rounds that every lane runs in double precision (some lane of a wave nearly
-// always has a pattern of five or six bits, so a wave-uniform skip of the later
-// rounds measured slower: it kept the rounds' loads from overlapping); a lane
-// whose pattern has fewer bits reads the null record (row 0), which leaves both
-// sums unchanged. Bits past the sixth (rare) come from the mask. The path metrics
-// add the increments in ascending sorted position, as getPM does (:765).
+// ends the search). The sums of the first six bits of each pattern come from
+// pattern_sums; bits past the sixth (rare) come from the mask.
 //
 // Under a guess budget the guard T < Tmax runs once per weight level (:652),
 // before the first pattern of each level (the words' level bit), with T = t + 1.
@@ -708,42 +775,9 @@ __device__ __forceinline__ void search_pair(SearchState &ss, LineList &ls, const
         if (take2 && (w2 & kPosLevel) && !((uint64_t)t + 2u < cfg.tmax)) take2 = false;
     }
     if (!take2) w2 = 0;
-    const Rec pm_base = (Rec)pm_hd;
     Rec pm1 = 0, pm2 = 0;
     uint32_t syn1, syn2;
-    {
-        const uint32_t a1 = pos_field<0>(w1), b1 = pos_field<1>(w1);
-        const uint32_t a2 = pos_field<0>(w2), b2 = pos_field<1>(w2);
-        syn1 = xor3(s0, (uint32_t)S.hr(a1), (uint32_t)S.hr(b1));
-        syn2 = xor3(s0, (uint32_t)S.hr(a2), (uint32_t)S.hr(b2));
-        if constexpr (kPmLoop) {
-            pm1 = (pm_base + S.dr(a1)) + S.dr(b1);
-            pm2 = (pm_base + S.dr(a2)) + S.dr(b2);
-        }
-    }
-    auto round = [&](auto i0) {
-        constexpr int I = decltype(i0)::value;
-        const uint32_t a1 = pos_field<I>(w1), b1 = pos_field<I + 1>(w1);
-        const uint32_t a2 = pos_field<I>(w2), b2 = pos_field<I + 1>(w2);
-        syn1 = xor3(syn1, (uint32_t)S.hr(a1), (uint32_t)S.hr(b1));
-        syn2 = xor3(syn2, (uint32_t)S.hr(a2), (uint32_t)S.hr(b2));
-        if constexpr (kPmLoop) {
-            pm1 = (pm1 + S.dr(a1)) + S.dr(b1);
-            pm2 = (pm2 + S.dr(a2)) + S.dr(b2);
-        }
-    };
-    if constexpr (P::kF32) {
-        // single-precision rounds are cheaper, and a wave-uniform skip of the
-        // later rounds measured 5 % faster there (fields 2.. and the overflow
-        // bit sit above bit 10, fields 4.. above bit 20)
-        if (__ballot(((w1 | w2) >> 11) != 0u) != 0ull) {
-            round(IntC<2>{});
-            if (__ballot(((w1 | w2) >> 21) != 0u) != 0ull) round(IntC<4>{});
-        }
-    } else {
-        round(IntC<2>{});
-        round(IntC<4>{});
-    }
+    pattern_sums<P, kPmLoop>(S, s0, pm_hd, w1, w2, pm1, pm2, syn1, syn2);
     const int tbl = ic < kSchedGenericIc ? ic : kSchedGenericIc;
     const uint16_t *masks = cfg.sched + (size_t)tbl * kSchedStride;
     if (__ballot(((w1 | w2) & kPosMore) != 0u) != 0ull) {
