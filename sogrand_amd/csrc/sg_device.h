@@ -656,10 +656,6 @@ __device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c)
     return r;
 }
 
-template <int I> struct IntC {
-    static constexpr int value = I;
-};
-
 // Field I of a position word (v_bfe_u32, written as the instruction: as a C
 // expression the compiler folds the extract into the address scaling, one
 // instruction more per field).
@@ -673,14 +669,13 @@ template <int I> __device__ __forceinline__ uint32_t pos_field(uint32_t w)
 // Path metrics and syndromes of the patterns of the position words w1 and w2
 // (fields 0..5: record rows, 0 = none): pm = PM_HD + the increments in
 // ascending sorted position, as getPM adds them (:765), and s0 XOR the H
-// columns. In double precision all twelve fields' records are read before the
-// sums: the reads do not depend on the sums, so their latencies overlap (3.8 %
-// faster than reading and summing two fields at a time); a lane whose pattern
-// has fewer bits reads the null record (row 0), which leaves both sums
-// unchanged. In single precision, whose rounds are cheaper, a round of two
-// fields per pattern runs only if some lane of the wave needs it (5 % faster
-// there). Without path metrics (the earlier decoder without a budget) only the
-// syndromes are formed.
+// columns. All twelve fields' records are read before the sums: the reads do
+// not depend on the sums, so their latencies overlap (reading and summing two
+// fields at a time measured 3.8 % slower in double and 10 % slower in single
+// precision, even with the single-precision rounds skipped wave-uniformly where
+// no lane needed them). A lane whose pattern has fewer bits reads the null
+// record (row 0), which leaves both sums unchanged. Without path metrics (the
+// earlier decoder without a budget) only the syndromes are formed.
 template <class P, bool kPmLoop, class LDS>
 __device__ __forceinline__ void pattern_sums(const LDS &S, uint32_t s0, double pm_hd, uint32_t w1,
                                              uint32_t w2, typename P::Rec &pm1,
@@ -688,58 +683,32 @@ __device__ __forceinline__ void pattern_sums(const LDS &S, uint32_t s0, double p
                                              uint32_t &syn2)
 {
     using Rec = typename P::Rec;
-    const Rec pm_base = (Rec)pm_hd;
-    if constexpr (!P::kF32 && kPmLoop) {
-        const uint32_t r1[6] = {pos_field<0>(w1), pos_field<1>(w1), pos_field<2>(w1),
-                                pos_field<3>(w1), pos_field<4>(w1), pos_field<5>(w1)};
-        const uint32_t r2[6] = {pos_field<0>(w2), pos_field<1>(w2), pos_field<2>(w2),
-                                pos_field<3>(w2), pos_field<4>(w2), pos_field<5>(w2)};
-        Rec d1[6], d2[6];
-        uint32_t h1[6], h2[6];
+    const uint32_t r1[6] = {pos_field<0>(w1), pos_field<1>(w1), pos_field<2>(w1),
+                            pos_field<3>(w1), pos_field<4>(w1), pos_field<5>(w1)};
+    const uint32_t r2[6] = {pos_field<0>(w2), pos_field<1>(w2), pos_field<2>(w2),
+                            pos_field<3>(w2), pos_field<4>(w2), pos_field<5>(w2)};
+    Rec d1[6], d2[6];
+    uint32_t h1[6], h2[6];
 #pragma unroll
-        for (int k = 0; k < 6; ++k) {
+    for (int k = 0; k < 6; ++k) {
+        if constexpr (kPmLoop) {
             d1[k] = S.dr(r1[k]);
             d2[k] = S.dr(r2[k]);
-            h1[k] = S.hr(r1[k]);
-            h2[k] = S.hr(r2[k]);
         }
-        pm1 = pm_base;
-        pm2 = pm_base;
+        h1[k] = S.hr(r1[k]);
+        h2[k] = S.hr(r2[k]);
+    }
+    pm1 = (Rec)pm_hd;
+    pm2 = (Rec)pm_hd;
+    if constexpr (kPmLoop) {
 #pragma unroll
         for (int k = 0; k < 6; ++k) {
             pm1 = pm1 + d1[k];
             pm2 = pm2 + d2[k];
         }
-        syn1 = xor3(xor3(xor3(s0, h1[0], h1[1]), h1[2], h1[3]), h1[4], h1[5]);
-        syn2 = xor3(xor3(xor3(s0, h2[0], h2[1]), h2[2], h2[3]), h2[4], h2[5]);
-    } else {
-        auto round = [&](auto i0) {
-            constexpr int I = decltype(i0)::value;
-            const uint32_t a1 = pos_field<I>(w1), b1 = pos_field<I + 1>(w1);
-            const uint32_t a2 = pos_field<I>(w2), b2 = pos_field<I + 1>(w2);
-            syn1 = xor3(syn1, (uint32_t)S.hr(a1), (uint32_t)S.hr(b1));
-            syn2 = xor3(syn2, (uint32_t)S.hr(a2), (uint32_t)S.hr(b2));
-            if constexpr (kPmLoop) {
-                pm1 = (pm1 + S.dr(a1)) + S.dr(b1);
-                pm2 = (pm2 + S.dr(a2)) + S.dr(b2);
-            }
-        };
-        syn1 = s0;
-        syn2 = s0;
-        pm1 = pm_base;
-        pm2 = pm_base;
-        round(IntC<0>{});
-        if constexpr (P::kF32) {
-            // fields 2.. (and the overflow bit) sit above bit 10, fields 4.. above bit 20
-            if (__ballot(((w1 | w2) >> 11) != 0u) != 0ull) {
-                round(IntC<2>{});
-                if (__ballot(((w1 | w2) >> 21) != 0u) != 0ull) round(IntC<4>{});
-            }
-        } else {
-            round(IntC<2>{});
-            round(IntC<4>{});
-        }
     }
+    syn1 = xor3(xor3(xor3(s0, h1[0], h1[1]), h1[2], h1[3]), h1[4], h1[5]);
+    syn2 = xor3(xor3(xor3(s0, h2[0], h2[1]), h2[2], h2[3]), h2[4], h2[5]);
 }
 
 // Two consecutive guesses of the schedule (:652-744): the patterns of the
