@@ -307,11 +307,13 @@ def workload_uneven(args) -> dict:
     d_bits = hip.DeviceArray((n, 64), np.uint8)
     d_st = hip.DeviceArray((n,), sg.CW_STATS_DTYPE)
     info = sg.gen_info_bits(args.seed, n)
-    # launch shape of the persistent kernel: autotuned on a sample of this workload
+    # launch shape of the persistent kernel: autotuned on a sample of this workload,
+    # the pick confirmed against the default shape on the whole of it
     from sogrand_amd.autotune import autotune
 
     n_tune = min(n, 32768)
-    tuned = autotune(dec, _View(d_llr, 0, (n_tune, 4096)), n_tune, log=log)
+    tuned = autotune(dec, _View(d_llr, 0, (n_tune, 4096)), n_tune, log=log,
+                     confirm=(d_llr, n))
     log(f"autotuned shape {tuned['best']}: {tuned['speedup_vs_default']}x the default")
     rows = []
     ref_bits = None

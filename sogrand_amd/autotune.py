@@ -7,7 +7,10 @@ gathers before handing out new lines (`lane_batch`), how many guesses a lane
 runs between scheduling points (`step_len`) and how many waves per compute
 unit stay resident (`waves_per_cu`). The tuner times the kernel on a sample of
 the caller's workload (device-resident) by coordinate descent over a grid of
-those values and returns the fastest shape.
+those values and returns the fastest shape. A sample holds only a few cubes per
+resident cube slot, so the end of the batch, where slots run dry, weighs more
+in it than in a large workload; given the whole workload (`confirm`), the tuner
+times its pick and the default shape on it once each and keeps the faster.
 """
 from __future__ import annotations
 
@@ -40,11 +43,14 @@ def _time(dec, d_llr, d_bits, d_st, shape, repeats):
     return best
 
 
-def autotune(dec, d_llr, n: int, rounds: int = 2, repeats: int = 2, log=None) -> dict:
+def autotune(dec, d_llr, n: int, rounds: int = 2, repeats: int = 2, log=None,
+             confirm=None) -> dict:
     """Tune the persistent kernel's shape on n device-resident codewords.
 
-    Returns {"best": shape, "best_ms": t, "default_ms": t0, "trials": [...]} and
-    leaves `dec` set to the best shape (backend persistent)."""
+    confirm: optional (d_llr_all, n_all), the whole workload the shape is for.
+    Returns {"best": shape, "best_ms": t, "default_ms": t0, "trials": [...]}
+    (plus "confirm": the two whole-workload times, if it ran) and leaves `dec`
+    set to the best shape (backend persistent)."""
     import sogrand_amd as sg
     from sogrand_amd import hip
 
@@ -76,11 +82,27 @@ def autotune(dec, d_llr, n: int, rounds: int = 2, repeats: int = 2, log=None) ->
                     best, best_ms, improved = cand, ms, True
         if not improved:
             break
-    dec.set_params(**best)
     d_bits.free()
     d_st.free()
-    return {"best": best, "best_ms": round(best_ms, 3), "default_ms": round(t_default, 3),
-            "speedup_vs_default": round(t_default / best_ms, 4), "n_cw": n, "trials": trials}
+    out = {"best": best, "best_ms": round(best_ms, 3), "default_ms": round(t_default, 3),
+           "speedup_vs_default": round(t_default / best_ms, 4), "n_cw": n, "trials": trials}
+    if confirm is not None and best != DEFAULT:
+        d_all, n_all = confirm
+        a_bits = hip.DeviceArray((n_all, 64), np.uint8)
+        a_st = hip.DeviceArray((n_all,), sg.CW_STATS_DTYPE)
+        ms_best = _time(dec, d_all, a_bits, a_st, best, 1)
+        ms_def = _time(dec, d_all, a_bits, a_st, DEFAULT, 1)
+        a_bits.free()
+        a_st.free()
+        out["confirm"] = {"n_cw": n_all, "tuned_ms": round(ms_best, 3),
+                          "default_ms": round(ms_def, 3)}
+        if log:
+            log(f"autotune confirm on {n_all} codewords: {best} {ms_best:.1f} ms, "
+                f"default {ms_def:.1f} ms")
+        if ms_def <= ms_best:
+            out["best"] = best = dict(DEFAULT)
+    dec.set_params(**best)
+    return out
 
 
 def save(result: dict, path: str) -> None:

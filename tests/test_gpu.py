@@ -237,6 +237,35 @@ def test_launch_shapes_match_oracle(decoder, oracle, shape):
     assert soft_close(r.soft, soft), soft_err(r.soft, soft)
 
 
+def test_autotune_confirms_its_pick_on_the_whole_workload(decoder):
+    # the tuner checks that every shape it tries decodes identically; given the
+    # whole workload it times its pick against the default shape there and keeps
+    # the faster, leaving the decoder on the shape it returns
+    sg = _sg()
+    from sogrand_amd import hip
+    from sogrand_amd.autotune import DEFAULT, autotune
+
+    n = 4096
+    ebn0 = np.random.default_rng(12).uniform(0.5, 3.5, n)
+    llr = sg.gen_llr(1212, n, ebn0_db=ebn0)
+    d_llr = hip.DeviceArray.from_host(llr)
+    d_sample = hip.DeviceArray.from_host(llr[:1024])
+    backend = decoder.params.backend
+    try:
+        r = autotune(decoder, d_sample, 1024, rounds=1, repeats=1, confirm=(d_llr, n))
+        got = {k: getattr(decoder.params, k) for k in DEFAULT}
+    finally:
+        decoder.set_params(backend=backend, wave_slots=0, lane_batch=0, step_len=0,
+                           waves_per_cu=0)
+        d_llr.free()
+        d_sample.free()
+    assert got == r["best"]
+    if r["best"] != DEFAULT:
+        assert r["confirm"]["n_cw"] == n
+        assert r["confirm"]["tuned_ms"] < r["confirm"]["default_ms"]
+    _say("autotune", r["best"], r.get("confirm"))
+
+
 def test_generator_device_equals_host(decoder):
     sg = _sg()
     from sogrand_amd import hip
