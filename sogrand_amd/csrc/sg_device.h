@@ -20,6 +20,7 @@
 #include <stdint.h>
 
 #include "sg_common.h"
+#include "sg_conf.h"
 #include "sg_exp.h"
 #include "sg_log.h"
 
@@ -41,11 +42,13 @@ struct DecodeCfg {
                             // cubic_decoder.c:533)
 };
 
-// Entries of the LDS table: the exp table, then one entry {eps, thres} that the
-// list insertion reads where it needs them (as a broadcast load, which keeps
-// the two doubles out of the scalar registers the search loop runs short of),
-// then the log's polynomial coefficients (kLogCoef, two per entry).
-constexpr int kLdsLogCoefEntry = kExpTableEntries + 1;
+// Entries of the LDS table: the exp table, then the entries {eps, thres} and
+// {conf_gap(thres, eps), 0} that the list insertion reads where it needs them
+// (as broadcast loads, which keep the doubles out of the scalar registers the
+// search loop runs short of), then the log's polynomial coefficients
+// (kLogCoef, two per entry).
+constexpr int kLdsGapEntry = kExpTableEntries + 1;
+constexpr int kLdsLogCoefEntry = kExpTableEntries + 2;
 constexpr int kLdsTableEntries = kLdsLogCoefEntry + kLogCoefs / 2;
 
 // Fill the LDS table (first kLdsTableEntries threads) - the caller synchronises.
@@ -55,6 +58,8 @@ __device__ __forceinline__ void load_exp_table(ExpEntry *lds, int tid, double ep
         lds[tid] = exp_table_entry(tid);
     } else if (tid == kExpTableEntries) {
         lds[tid] = ExpEntry{eps, thres};
+    } else if (tid == kLdsGapEntry) {
+        lds[tid] = ExpEntry{conf_gap(thres, eps), 0.0};
     } else if (tid < kLdsTableEntries) {
         const int j = 2 * (tid - kLdsLogCoefEntry);
         lds[tid] = ExpEntry{kLogCoef[j], kLogCoef[j + 1]};
@@ -154,13 +159,22 @@ __device__ __forceinline__ double exp_neg_f32(float pm)
 // natural position << 8) of sorted positions b = 0..15 at rows 1..16, and at
 // row 0 a null record (delta +0.0, H column 0) that the pattern rounds read
 // for the empty fields (row 0) of a position word. The column pointers fold the
-// slot in, so a record address is one shift-add of the row.
+// slot in, so a record address is one shift-add of the row. Two 2-byte H
+// records share a 4-byte LDS word; the pair that shares one is slots s and
+// s + 32 of each group of 64 (column h_column(s)), which sit in different
+// halves of a wave, so the lanes of one half never meet on a bank when they
+// read different rows.
+__host__ __device__ constexpr int h_column(int slot)
+{
+    return (slot & ~63) | ((slot & 31) << 1) | ((slot >> 5) & 1);
+}
 template <class RecT, int NS> struct LineLdsT {
+    static_assert(NS % 64 == 0, "record arrays hold whole waves");
     using Rec = RecT;
     Rec *dcol;       // this slot's record of sorted position 0
     uint16_t *hcol;
     __device__ LineLdsT(Rec *delta, uint16_t *hp, int slot)
-        : dcol(delta + NS + slot), hcol(hp + NS + slot)
+        : dcol(delta + NS + slot), hcol(hp + NS + h_column(slot))
     {
     }
     __device__ Rec &d(int b) const { return dcol[b * NS]; }
@@ -460,10 +474,11 @@ __device__ __forceinline__ bool list_hit(LineList &ls, double e, uint32_t code, 
     const double pneg = 0.00390625 * pnot; // 2^-(n-k) * P_notGuess
     ls.pnl = pneg;
     const ExpEntry et = cfg.exptab[kExpTableEntries]; // {eps, thres}
-    const double conf = (ls.ppos + pneg > et.a) ? ls.ppos / (ls.ppos + pneg) : 1.0;
+    const double gap = cfg.exptab[kLdsGapEntry].a;
+    const bool confident = conf_above(ls.ppos, pneg, et.a, et.b, gap);
     ls.code |= (uint64_t)code << (16 * ls.nl);
     ls.nl += 1;
-    return conf > et.b || (uint32_t)ls.nl == cfg.L;
+    return confident || (uint32_t)ls.nl == cfg.L;
 }
 
 // Entry q (< ls.nl) of a line's list, rebuilt from its schedule position with

@@ -41,6 +41,23 @@ struct ExpParts {
     uint64_t ki;    // low word: k
 };
 
+// Scale bits T[j] bits + (k << 45). The addend's low word is zero, so only the
+// high words add: on the device one 32-bit shift-add written as the
+// instruction (as C, the compiler rebuilds a 64-bit add with register moves).
+__host__ __device__ inline uint64_t scale_bits(double tb, uint64_t ki)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    struct Words {
+        uint32_t lo, hi;
+    };
+    Words w = __builtin_bit_cast(Words, tb);
+    asm("v_lshl_add_u32 %0, %1, 13, %0" : "+v"(w.hi) : "v"((uint32_t)ki));
+    return __builtin_bit_cast(uint64_t, w);
+#else
+    return __builtin_bit_cast(uint64_t, tb) + (ki << 45);
+#endif
+}
+
 __host__ __device__ inline ExpParts exp_parts(double x, const ExpEntry *tab)
 {
     const double kd0 = __builtin_fma(x, kExpInvLn2N, kExpShift);
@@ -52,7 +69,7 @@ __host__ __device__ inline ExpParts exp_parts(double x, const ExpEntry *tab)
     const double r2 = r * r;
     const double tmp = __builtin_fma(r2 * r2, __builtin_fma(r, c54.a, c54.b),
                                      __builtin_fma(r2, __builtin_fma(r, c32.a, c32.b), t.a + r));
-    return {tmp, __builtin_bit_cast(uint64_t, t.b) + (ki << 45), ki};
+    return {tmp, scale_bits(t.b, ki), ki};
 }
 
 // |x| < 512: scale + scale tmp, one fused multiply-add.
