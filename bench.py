@@ -180,7 +180,7 @@ def wilson(k: int, n: int, z: float = 1.96) -> tuple[float, float]:
 
 def serve_load(dec, capacity: float, load: float, seconds: float, seed: int,
                max_inflight: int = 16, max_batch_cw: int = 0, plain_below_cw: int = 0,
-               hold_above_cw: int = 0) -> dict:
+               hold_above_cw: int = 0, lanegroup_max_cw: int = 0) -> dict:
     """Open-loop Poisson arrivals of frames (U{1..256} codewords, per-frame Eb/N0
     ~ U[1, 3] dB) at `load` x capacity, through the serving loop. Latency runs
     from a frame's scheduled arrival to its results being back in host memory."""
@@ -205,7 +205,8 @@ def serve_load(dec, capacity: float, load: float, seconds: float, seed: int,
     views = [_View(d_llr, int(offs[k]) * 4096 * 8, (int(sizes[k]), 4096)) for k in range(nf)]
     done = []
     with sg.Server(dec, hbm_budget=8 << 30, max_inflight=max_inflight, max_batch_cw=max_batch_cw,
-                   plain_below_cw=plain_below_cw, hold_above_cw=hold_above_cw) as srv:
+                   plain_below_cw=plain_below_cw, hold_above_cw=hold_above_cw,
+                   lanegroup_max_cw=lanegroup_max_cw) as srv:
         t0 = srv.clock() + 0.01
         lag = 0.0
         for k in range(nf):
@@ -278,7 +279,7 @@ def workload_serving(args) -> dict:
     for load in (0.1, 0.3, 0.5, 0.7, 0.9, 1.0, 1.2):
         r = serve_load(dec, cap, load, args.serve_seconds, args.seed + int(load * 100),
                        args.serve_slots, args.serve_batch, args.serve_plain_below,
-                       args.serve_hold_above)
+                       args.serve_hold_above, args.serve_lanegroup_max)
         log(f"load {load:.1f}: {r}")
         pts.append(r)
     return {"metric": "serving: p50/p99 frame latency and sustained codewords/s vs offered load",
@@ -288,7 +289,8 @@ def workload_serving(args) -> dict:
                        "seconds_per_point": args.serve_seconds, "slots": args.serve_slots,
                        "max_batch_cw": args.serve_batch,
                        "plain_below_cw": args.serve_plain_below,
-                       "hold_above_cw": args.serve_hold_above},
+                       "hold_above_cw": args.serve_hold_above,
+                       "lanegroup_max_cw": args.serve_lanegroup_max},
             "capacity_cw_s": round(cap, 1), "points": pts}
 
 
@@ -545,6 +547,46 @@ def workload_square(args) -> dict:
 # HBM counter calibration
 
 
+def workload_latency(args) -> dict:
+    """Time to decode one small frame alone, per backend: frames of 1-4096
+    codewords at 1 and 2 dB, device-resident, each decoded three times (the
+    fastest kernel time, HIP events on the launch stream); bits and statistics
+    checked identical across the backends."""
+    import sogrand_amd as sg
+    from sogrand_amd import hip
+
+    dec = sg.Decoder(0)
+    names = ((sg.BACKEND_PLAIN, "plain"), (sg.BACKEND_PERSISTENT, "persistent"),
+             (sg.BACKEND_LANEGROUP, "lanegroup"))
+    rows = []
+    for ebn0 in (1.0, 2.0):
+        for n in (1, 4, 16, 64, 256, 1024, 4096):
+            llr = gen_device(dec, args.seed + n, n, sg.sigma_for_ebn0(ebn0))
+            bits = hip.DeviceArray((n, 64), np.uint8)
+            st = hip.DeviceArray((n,), sg.CW_STATS_DTYPE)
+            row = {"ebn0_db": ebn0, "n_cw": n}
+            ref = None
+            for b, name in names:
+                dec.set_params(backend=b)
+                best = None
+                for _ in range(3):
+                    dec.decode(llr, out_bits=bits, out_stats=st)
+                    ms = dec.last_kernel_ms()[0]
+                    best = ms if best is None else min(best, ms)
+                got = (bits.to_host().copy(), st.to_host().copy())
+                if ref is None:
+                    ref = got
+                elif not ((got[0] == ref[0]).all() and (got[1] == ref[1]).all()):
+                    raise RuntimeError(f"backend {name} differs at {n} codewords, {ebn0} dB")
+                row[name + "_ms"] = round(best, 3)
+            log(f"{ebn0} dB, {n} cw: {row}")
+            rows.append(row)
+    return {"metric": "frame decode time by backend (kernel, HIP events)", "unit": "ms",
+            "n_gpus": 1, "dtype": "f64", "data": "synthetic (seeded AWGN frames)",
+            "config": {"workload": "one frame alone, 1-4096 codewords at 1 and 2 dB, FP64"},
+            "identical_across_backends": True, "rows": rows}
+
+
 def workload_copy(args) -> dict:
     """Three device-to-device copies of 1 GiB: a kernel of known traffic (1 GiB
     read + 1 GiB written each) that scales the FETCH_SIZE / WRITE_SIZE counters."""
@@ -702,7 +744,8 @@ def workload_headline(args) -> dict:
     if not args.no_serve:
         cap = serving_capacity(dec, args.seed + 1)
         r = serve_load(dec, cap, 0.7, args.serve_seconds, args.seed + 2, args.serve_slots,
-                       args.serve_batch, args.serve_plain_below, args.serve_hold_above)
+                       args.serve_batch, args.serve_plain_below, args.serve_hold_above,
+                       args.serve_lanegroup_max)
         log(f"serving at 70 % of {cap:.0f} cw/s: {r}")
         out["latency"] = {"p50_ms": r["p50_ms"], "p99_ms": r["p99_ms"],
                           "workload": "serving loop, Poisson frames of U{1..256} codewords at "
@@ -723,7 +766,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="headline",
                     choices=["headline", "snr-sweep", "uneven", "mixed-precision", "serving", "square",
-                             "variants", "copy"])
+                             "variants", "latency", "copy"])
     ap.add_argument("--n-cw", type=int, default=None)
     ap.add_argument("--ebn0", type=float, default=2.0)
     ap.add_argument("--seed", type=int, default=20260)
@@ -733,6 +776,9 @@ def main() -> int:
     ap.add_argument("--serve-batch", type=int, default=0)
     ap.add_argument("--serve-plain-below", type=int, default=0)
     ap.add_argument("--serve-hold-above", type=int, default=0)
+    ap.add_argument("--serve-lanegroup-max", type=int, default=0,
+                    help="a batch runs the lane-group kernel while it and the batches still "
+                         "decoding hold at most this many codewords (0 = never)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
@@ -748,7 +794,7 @@ def main() -> int:
     fn = {"headline": workload_headline, "uneven": workload_uneven,
           "mixed-precision": workload_mixed, "serving": workload_serving, "square": workload_square,
           "variants": workload_variants, "snr-sweep": workload_snr_sweep,
-          "copy": workload_copy}[args.workload]
+          "latency": workload_latency, "copy": workload_copy}[args.workload]
     print(json.dumps(fn(args)), flush=True)
     return 0
 

@@ -57,6 +57,10 @@ extern "C" {
 /* backend */
 #define SOGRAND_BACKEND_PLAIN 0      /* one workgroup per codeword, static lines */
 #define SOGRAND_BACKEND_PERSISTENT 1 /* persistent waves + device queue          */
+#define SOGRAND_BACKEND_LANEGROUP 2  /* one 1024-lane workgroup per codeword, the
+                                        cube in LDS, four lanes per line: the
+                                        shortest time per codeword (small frames);
+                                        FP64 with the final variant only         */
 
 /* Decoder parameters. Defaults (sogrand_params_default) are the reference's
  * constants: L=3, Imax=30, Tmax=UINT64_MAX, p_ET=1e-5, alpha=0.7, EPSILON=1e-40
@@ -225,6 +229,11 @@ typedef struct sogrand_server_cfg {
                                 decoding, so that waiting requests leave together as one
                                 larger batch (0 = no limit). A batch completes with its
                                 slowest cube, so holding costs throughput on uneven work  */
+    uint32_t lanegroup_max_cw; /* a batch runs the lane-group kernel (cube in LDS, four
+                                lanes per line: a codeword finishes soonest) while it and
+                                every batch still decoding hold at most this many
+                                codewords; FP64 final variant only (default 0 = never:
+                                at 70 % load its CU-sized workgroups lengthen the tail)    */
 } sogrand_server_cfg;
 
 typedef struct sogrand_completion {

@@ -37,6 +37,7 @@ VARIANT_FINAL = 0   # SOGRAND_C/cubic_decoder_final.c (and _updated.c via epsilo
 VARIANT_LEGACY = 1  # SOGRAND_C/cubic_decoder.c, the earlier decoder (FP64 only)
 BACKEND_PLAIN = 0
 BACKEND_PERSISTENT = 1
+BACKEND_LANEGROUP = 2  # cube in LDS, four lanes per line (FP64, final variant)
 UINT64_MAX = (1 << 64) - 1
 
 CW_STATS_DTYPE = np.dtype(
@@ -93,6 +94,7 @@ class ServerCfg(C.Structure):
         ("with_soft", C.c_int32),
         ("plain_below_cw", C.c_uint32),
         ("hold_above_cw", C.c_uint32),
+        ("lanegroup_max_cw", C.c_uint32),
     ]
 
 
@@ -325,7 +327,7 @@ class Server:
 
     def __init__(self, decoder: Decoder, hbm_budget: int | None = None, max_inflight: int = 16,
                  max_batch_cw: int = 0, max_pending_cw: int = 1 << 20, with_soft: bool = False,
-                 plain_below_cw: int = 0, hold_above_cw: int = 0):
+                 plain_below_cw: int = 0, hold_above_cw: int = 0, lanegroup_max_cw: int = 0):
         lib = decoder._lib
         self._lib = lib
         cfg = ServerCfg()
@@ -338,6 +340,7 @@ class Server:
         cfg.with_soft = 1 if with_soft else 0
         cfg.plain_below_cw = plain_below_cw
         cfg.hold_above_cw = hold_above_cw
+        cfg.lanegroup_max_cw = lanegroup_max_cw
         h = C.c_void_p()
         _check(lib.sogrand_server_create(C.byref(h), decoder._h, C.byref(cfg)), "server_create")
         self._h = h

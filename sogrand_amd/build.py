@@ -36,8 +36,8 @@ COMMON = [
 ]
 
 LIB_SOURCES = [
-    "sg_kernels.hip", "sg_persistent.hip", "sg_api.cpp", "sg_schedule.cpp", "sg_serve.cpp",
-    "sg_noalloc.hip",
+    "sg_kernels.hip", "sg_persistent.hip", "sg_lanegroup.hip", "sg_api.cpp", "sg_schedule.cpp",
+    "sg_serve.cpp", "sg_noalloc.hip",
 ]
 
 

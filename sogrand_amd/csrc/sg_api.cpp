@@ -58,7 +58,13 @@ LaunchPlan plan_launch(const sogrand_ctx *ctx, const sogrand_params &p, size_t n
 {
     LaunchPlan lp{};
     lp.persistent = p.backend == SOGRAND_BACKEND_PERSISTENT;
-    if (lp.persistent) {
+    lp.lanegroup = p.backend == SOGRAND_BACKEND_LANEGROUP;
+    if (lp.lanegroup) {
+        // the cube lives in LDS: no L_E scratch
+        lp.blocks = (int)lanegroup_blocks(n, ctx->cu_count);
+        lp.slots = 1;
+        lp.le_slots = 0;
+    } else if (lp.persistent) {
         const int slots = p.wave_slots > 0 ? p.wave_slots : 2;
         const int wpb = kPersistentWaves;
         // a persistent grid must be resident all at once: never more workgroups
@@ -113,6 +119,7 @@ hipError_t launch_cubes(const sogrand_params &p, const LaunchPlan &lp, const Cub
         return launch_persistent(job, kPersistentWaves, lp.blocks, lp.slots, batch, step, st);
     }
     if (ev_start) (void)hipEventRecord(ev_start, st);
+    if (lp.lanegroup) return launch_lanegroup(job, lp.blocks, st);
     return launch_plain(job, lp.blocks, st);
 }
 
@@ -163,7 +170,12 @@ static int check_params(const sogrand_params *p)
     if (!(p->p_ET >= 0.0 && p->p_ET <= 1.0)) return SOGRAND_EINVAL;
     if (!std::isfinite(p->alpha)) return SOGRAND_EINVAL;
     if (p->precision != SOGRAND_FP64 && p->precision != SOGRAND_FP32_MIXED) return SOGRAND_EINVAL;
-    if (p->backend != SOGRAND_BACKEND_PLAIN && p->backend != SOGRAND_BACKEND_PERSISTENT)
+    if (p->backend != SOGRAND_BACKEND_PLAIN && p->backend != SOGRAND_BACKEND_PERSISTENT &&
+        p->backend != SOGRAND_BACKEND_LANEGROUP)
+        return SOGRAND_EINVAL;
+    // the lane-group kernel has the final decoder's double-precision arithmetic only
+    if (p->backend == SOGRAND_BACKEND_LANEGROUP &&
+        (p->precision != SOGRAND_FP64 || p->variant != SOGRAND_VARIANT_FINAL))
         return SOGRAND_EINVAL;
     if (p->wave_slots < 0 || p->wave_slots > 4 || p->waves_per_cu < 0 || p->waves_per_cu > 32)
         return SOGRAND_EINVAL;

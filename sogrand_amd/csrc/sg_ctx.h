@@ -92,6 +92,7 @@ namespace sg {
 // Grid and L_E scratch of one decode launch of n codewords.
 struct LaunchPlan {
     bool persistent;
+    bool lanegroup;
     int blocks;
     int slots;       // persistent: cube slots per wave
     size_t le_slots; // 4096-f64 L_E slots the launch needs

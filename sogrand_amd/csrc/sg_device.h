@@ -163,26 +163,29 @@ __device__ __forceinline__ double exp_neg_f32(float pm)
 // records share a 4-byte LDS word; the pair that shares one is slots s and
 // s + 32 of each group of 64 (column h_column(s)), which sit in different
 // halves of a wave, so the lanes of one half never meet on a bank when they
-// read different rows.
+// read different rows. RS is the row stride in records: NS where each lane
+// reads its own slot, NS + 1 where several lanes read different rows of one
+// slot (one record of padding per row moves each row to the next bank).
 __host__ __device__ constexpr int h_column(int slot)
 {
     return (slot & ~63) | ((slot & 31) << 1) | ((slot >> 5) & 1);
 }
-template <class RecT, int NS> struct LineLdsT {
+template <class RecT, int NS, int RS = NS> struct LineLdsT {
     static_assert(NS % 64 == 0, "record arrays hold whole waves");
+    static_assert(RS >= NS, "rows hold every slot");
     using Rec = RecT;
     Rec *dcol;       // this slot's record of sorted position 0
     uint16_t *hcol;
     __device__ LineLdsT(Rec *delta, uint16_t *hp, int slot)
-        : dcol(delta + NS + slot), hcol(hp + NS + h_column(slot))
+        : dcol(delta + RS + slot), hcol(hp + RS + h_column(slot))
     {
     }
-    __device__ Rec &d(int b) const { return dcol[b * NS]; }
-    __device__ uint16_t &h(int b) const { return hcol[b * NS]; }
+    __device__ Rec &d(int b) const { return dcol[b * RS]; }
+    __device__ uint16_t &h(int b) const { return hcol[b * RS]; }
     // the same records by row (row = b + 1, row 0 the null record), as the
     // fields of a position word give them
-    __device__ Rec dr(uint32_t row) const { return dcol[((int)row - 1) * NS]; }
-    __device__ uint16_t hr(uint32_t row) const { return hcol[((int)row - 1) * NS]; }
+    __device__ Rec dr(uint32_t row) const { return dcol[((int)row - 1) * RS]; }
+    __device__ uint16_t hr(uint32_t row) const { return hcol[((int)row - 1) * RS]; }
 };
 
 // JacLog (:762; cut at lim = 50, or 30 in cubic_decoder.c:699).

@@ -49,6 +49,11 @@ size_t plain_scratch_slots(size_t n_cw);
 hipError_t launch_square(const CubeJob &job, int grid, hipStream_t st);
 size_t square_blocks(size_t n_squares);
 
+// Lane-group backend: one 1024-lane workgroup per cube (cube in LDS, four lanes
+// per line), at most one resident workgroup per CU; FP64, final variant.
+hipError_t launch_lanegroup(const CubeJob &job, int grid, hipStream_t st);
+size_t lanegroup_blocks(size_t n_cw, int cu_count);
+
 // Persistent backend: waves pop cubes from a device counter; each wave keeps
 // `slots` cubes in flight and hands lines to lanes as they free up.
 hipError_t launch_persistent(const CubeJob &job, int waves_per_block, int blocks, int slots,

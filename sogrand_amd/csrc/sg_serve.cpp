@@ -75,6 +75,7 @@ struct sogrand_server {
     size_t batch_cw = 0;
     size_t plain_below = 0; // in-flight codewords below which batches run the plain kernel
     size_t hold_above = 0;  // no new batch while this many codewords are in flight
+    size_t lanegroup_max = 0; // batches up to this size run the lane-group kernel
     size_t inflight_cw = 0; // codewords of the batches on the device (worker thread only)
     std::vector<Slot> slots;
     std::deque<Request> pending;
@@ -159,6 +160,11 @@ void sogrand_server::launch(Slot &s)
     // that piled up while all slots were busy) run the configured kernel.
     // Both decode every codeword identically.
     if (n < plain_below) q.backend = SOGRAND_BACKEND_PLAIN;
+    // on a lightly loaded device (this batch and everything still decoding at
+    // most lanegroup_max codewords; off by default): the lane-group kernel,
+    // whose four lanes per line finish a codeword soonest. Its workgroup holds
+    // most of a CU's LDS, so under load it stops other batches from overlapping.
+    if (inflight_cw + n <= lanegroup_max) q.backend = SOGRAND_BACKEND_LANEGROUP;
     inflight_cw += n;
     const sg::LaunchPlan lp = sg::plan_launch(ctx, q, n);
     ok = ok && sg::launch_cubes(q, lp, job, s.st, nullptr) == hipSuccess;
@@ -295,6 +301,7 @@ void sogrand_server_cfg_default(sogrand_server_cfg *c)
     c->with_soft = 0;
     c->plain_below_cw = 0;
     c->hold_above_cw = 0;
+    c->lanegroup_max_cw = 0;
 }
 
 int sogrand_server_create(sogrand_server **out, sogrand_ctx *ctx, const sogrand_server_cfg *cfg)
@@ -331,6 +338,9 @@ int sogrand_server_create(sogrand_server **out, sogrand_ctx *ctx, const sogrand_
     s->batch_cw = b;
     s->plain_below = c.plain_below_cw ? c.plain_below_cw : (size_t)3 * ctx->cu_count;
     s->hold_above = c.hold_above_cw ? c.hold_above_cw : SIZE_MAX;
+    // the lane-group kernel has the final decoder's FP64 arithmetic only
+    const bool lg_ok = s->p.precision == SOGRAND_FP64 && s->p.variant == SOGRAND_VARIANT_FINAL;
+    s->lanegroup_max = lg_ok ? (size_t)c.lanegroup_max_cw : 0;
     if (hipSetDevice(ctx->device) != hipSuccess) {
         delete s;
         return SOGRAND_EHIP;

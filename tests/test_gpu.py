@@ -63,7 +63,7 @@ def test_lines_random_match_oracle(decoder, oracle):
         assert soft_close(le[i], le2), (i, soft_err(le[i], le2))
 
 
-@pytest.mark.parametrize("backend", [0, 1], ids=["plain", "persistent"])
+@pytest.mark.parametrize("backend", [0, 1, 2], ids=["plain", "persistent", "lanegroup"])
 @pytest.mark.parametrize("snr", SNRS)
 def test_cubes_match_reference_binary(decoder, oracle, backend, snr):
     sg = _sg()
@@ -111,7 +111,7 @@ def test_lines_with_large_metrics_match_oracle(decoder, oracle):
         assert soft_close(le[i], le2), (i, soft_err(le[i], le2))
 
 
-@pytest.mark.parametrize("backend", [0, 1], ids=["plain", "persistent"])
+@pytest.mark.parametrize("backend", [0, 1, 2], ids=["plain", "persistent", "lanegroup"])
 def test_cubes_with_large_llrs_match_oracle(decoder, oracle, backend):
     # channel LLRs scaled far beyond the channel's range (metrics above 512,
     # negative P_notGuess chains), whole decodes
@@ -125,7 +125,7 @@ def test_cubes_with_large_llrs_match_oracle(decoder, oracle, backend):
     assert soft_close(r.soft, soft), soft_err(r.soft, soft)
 
 
-@pytest.mark.parametrize("backend", [0, 1], ids=["plain", "persistent"])
+@pytest.mark.parametrize("backend", [0, 1, 2], ids=["plain", "persistent", "lanegroup"])
 def test_cubes_2000_match_oracle(decoder, oracle, backend):
     # the 2,000-codeword subset at 2 dB, checked against the oracle
     sg = _sg()
@@ -139,7 +139,7 @@ def test_cubes_2000_match_oracle(decoder, oracle, backend):
     assert soft_close(r.soft, soft), soft_err(r.soft, soft)
 
 
-@pytest.mark.parametrize("backend", [0, 1], ids=["plain", "persistent"])
+@pytest.mark.parametrize("backend", [0, 1, 2], ids=["plain", "persistent", "lanegroup"])
 def test_cubes_with_ties_match_oracle(decoder, oracle, backend):
     # channel LLRs on a grid of 1/4: most lines start with equal magnitudes, so
     # the set-up's tie detection and the quicksort replay run in every kernel
@@ -162,11 +162,13 @@ def test_backends_identical_and_batch_independent(decoder):
     snr = rng.uniform(0.5, 3.5, 600)
     llr = sg.gen_llr(777, 600, ebn0_db=snr)
     out = []
-    for backend in (0, 1):
+    for backend in (0, 1, 2):
         decoder.set_params(backend=backend)
         out.append(decoder.decode(llr, soft=True))
-    a, b = out
-    assert (a.bits == b.bits).all() and (a.stats == b.stats).all() and (a.soft == b.soft).all()
+    a = out[0]
+    for b in out[1:]:
+        assert (a.bits == b.bits).all() and (a.stats == b.stats).all()
+        assert (a.soft == b.soft).all()
     # a codeword decodes the same alone, in a small batch and in the large one
     decoder.set_params(backend=1)
     for idx in ([17], [3, 400, 599], list(range(100, 164))):
@@ -183,7 +185,7 @@ def test_finite_tmax_matches_oracle(decoder, oracle, tmax):
     ebn0 = np.random.default_rng(tmax).uniform(0.5, 3.5, 150)
     llr = sg.gen_llr(31337, 150, ebn0_db=ebn0)
     bits, st, soft = oracle.decode(llr, Tmax=tmax, soft=True)
-    for backend in (0, 1):
+    for backend in (0, 1, 2):
         decoder.set_params(backend=backend, Tmax=tmax)
         r = decoder.decode(llr, soft=True)
         assert (r.bits == bits).all()
@@ -205,7 +207,7 @@ def test_smaller_lists_match_oracle(decoder, oracle, L):
     llr = sg.gen_llr(9090 + L, 96, ebn0_db=ebn0)
     bits, st, soft = oracle.decode(llr, L=L, soft=True)
     try:
-        for backend in (0, 1):
+        for backend in (0, 1, 2):
             decoder.set_params(backend=backend, L=L)
             r = decoder.decode(llr, soft=True)
             assert (r.bits == bits).all(), backend
@@ -370,7 +372,7 @@ def test_nonfinite_lines_match_reference(decoder):
     assert _same_or_close(le, g["lines_L_E"])
 
 
-@pytest.mark.parametrize("backend", [0, 1], ids=["plain", "persistent"])
+@pytest.mark.parametrize("backend", [0, 1, 2], ids=["plain", "persistent", "lanegroup"])
 def test_nonfinite_cubes_match_reference(decoder, backend):
     # cubes with a few +-inf entries (which the reference's channel can emit)
     # and a NaN: the reference runs all 30 iterations on them, as the NaN of
@@ -438,7 +440,7 @@ def test_cli_legacy_variant(tmp_path):
     assert lines[5] == f"Average NG_p per block: {g['NG_p_3.0'].sum() / n:.2f}"
 
 
-@pytest.mark.parametrize("backend", [0, 1], ids=["plain", "persistent"])
+@pytest.mark.parametrize("backend", [0, 1, 2], ids=["plain", "persistent", "lanegroup"])
 def test_graph_replays_match_eager(decoder, backend):
     # a captured launch sequence replayed several times, with eager decodes on
     # another context and host synchronisations in between; every replay must
@@ -482,7 +484,7 @@ def test_flow_script_end_to_end(tmp_path):
     assert (tmp_path / "decoded_cubic.bin").stat().st_size == 24 * 64
 
 
-@pytest.mark.parametrize("backend", [0, 1], ids=["plain", "persistent"])
+@pytest.mark.parametrize("backend", [0, 1, 2], ids=["plain", "persistent", "lanegroup"])
 def test_epsilon_variant_matches_reference_variant(decoder, oracle, backend):
     # the reference's DBL_EPSILON variant (cubic_decoder_updated.c): lines and
     # whole decodes to the same bar as the final decoder
@@ -556,5 +558,7 @@ def test_legacy_variant_matches_reference_legacy(decoder, oracle, backend):
             assert (np.abs(r.soft[:ns] - ref) <= 1e-10 * np.maximum(1.0, np.abs(ref))).all()
         with pytest.raises(Exception):
             decoder.set_params(precision=sg.FP32_MIXED)  # the earlier decoder is FP64 only
+        with pytest.raises(Exception):
+            decoder.set_params(backend=sg.BACKEND_LANEGROUP)  # final variant only
     finally:
         decoder.set_params(variant=sg.VARIANT_FINAL, Tmax=sg.UINT64_MAX, precision=sg.FP64)

@@ -34,8 +34,11 @@ def _frames(seed, n_frames, max_cw):
     return out
 
 
-@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "fp32"])
-def test_frames_identical_alone_and_in_the_mix(decoder, precision):
+@pytest.mark.parametrize("precision,lanegroup_max", [(0, 0), (1, 0), (0, 1 << 20), (1, 1 << 20)],
+                         ids=["fp64", "fp32", "fp64-lanegroup", "fp32-lanegroup"])
+def test_frames_identical_alone_and_in_the_mix(decoder, precision, lanegroup_max):
+    # lanegroup_max: batches run the lane-group kernel whenever the in-flight
+    # codewords allow (FP64; the FP32 server ignores it and runs its kernels)
     sg = _sg()
     from sogrand_amd import hip
 
@@ -47,7 +50,7 @@ def test_frames_identical_alone_and_in_the_mix(decoder, precision):
          "largest:", max(len(f) for f in frames))
     # batches of at most 384 codewords and four slots: several in flight at once
     with sg.Server(decoder, hbm_budget=2 << 30, max_inflight=4, max_batch_cw=384,
-                   with_soft=True) as srv:
+                   with_soft=True, lanegroup_max_cw=lanegroup_max) as srv:
         outs = []
         for k, f in enumerate(frames):
             n = len(f)

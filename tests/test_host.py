@@ -116,3 +116,25 @@ def test_decoder_without_gpu_fails_loudly():
     with pytest.raises(sg.SograndError) as e:
         sg.Decoder(0)
     assert e.value.code == sg.ENODEV
+
+
+def test_lanegroup_backend_parameter_checks():
+    # the lane-group backend has the final decoder's FP64 arithmetic only: the
+    # parameters are refused before any device is touched (so this runs without
+    # a GPU), while FP64 with the final variant (any epsilon) passes the checks
+    sg = _sg()
+    from sogrand_amd import hip
+
+    if hip.device_count() > 0:
+        pytest.skip("a GPU is present (the GPU tests cover the accepted cases)")
+    for kw in ({"precision": sg.FP32_MIXED}, {"variant": sg.VARIANT_LEGACY}):
+        with pytest.raises(sg.SograndError) as e:
+            sg.Decoder(0, backend=sg.BACKEND_LANEGROUP, **kw)
+        assert e.value.code == sg.EINVAL, kw
+    with pytest.raises(sg.SograndError) as e:
+        sg.Decoder(0, backend=3)
+    assert e.value.code == sg.EINVAL
+    for kw in ({}, {"epsilon": 2.220446049250313e-16}, {"Tmax": 1000}, {"L": 1}):
+        with pytest.raises(sg.SograndError) as e:
+            sg.Decoder(0, backend=sg.BACKEND_LANEGROUP, **kw)
+        assert e.value.code == sg.ENODEV, kw  # past the checks, then no device
