@@ -180,7 +180,8 @@ def wilson(k: int, n: int, z: float = 1.96) -> tuple[float, float]:
 
 def serve_load(dec, capacity: float, load: float, seconds: float, seed: int,
                max_inflight: int = 16, max_batch_cw: int = 0, plain_below_cw: int = 0,
-               hold_above_cw: int = 0, lanegroup_max_cw: int = 0) -> dict:
+               hold_above_cw: int = 0, lanegroup_max_cw: int = 0, small_batch_cw=None,
+               backlog_cw: int = 0) -> dict:
     """Open-loop Poisson arrivals of frames (U{1..256} codewords, per-frame Eb/N0
     ~ U[1, 3] dB) at `load` x capacity, through the serving loop. Latency runs
     from a frame's scheduled arrival to its results being back in host memory."""
@@ -206,7 +207,8 @@ def serve_load(dec, capacity: float, load: float, seconds: float, seed: int,
     done = []
     with sg.Server(dec, hbm_budget=8 << 30, max_inflight=max_inflight, max_batch_cw=max_batch_cw,
                    plain_below_cw=plain_below_cw, hold_above_cw=hold_above_cw,
-                   lanegroup_max_cw=lanegroup_max_cw) as srv:
+                   lanegroup_max_cw=lanegroup_max_cw, small_batch_cw=small_batch_cw,
+                   backlog_cw=backlog_cw) as srv:
         t0 = srv.clock() + 0.01
         lag = 0.0
         for k in range(nf):
@@ -279,7 +281,8 @@ def workload_serving(args) -> dict:
     for load in (0.1, 0.3, 0.5, 0.7, 0.9, 1.0, 1.2):
         r = serve_load(dec, cap, load, args.serve_seconds, args.seed + int(load * 100),
                        args.serve_slots, args.serve_batch, args.serve_plain_below,
-                       args.serve_hold_above, args.serve_lanegroup_max)
+                       args.serve_hold_above, args.serve_lanegroup_max, args.serve_small_batch,
+                       args.serve_backlog)
         log(f"load {load:.1f}: {r}")
         pts.append(r)
     return {"metric": "serving: p50/p99 frame latency and sustained codewords/s vs offered load",
@@ -290,7 +293,8 @@ def workload_serving(args) -> dict:
                        "max_batch_cw": args.serve_batch,
                        "plain_below_cw": args.serve_plain_below,
                        "hold_above_cw": args.serve_hold_above,
-                       "lanegroup_max_cw": args.serve_lanegroup_max},
+                       "lanegroup_max_cw": args.serve_lanegroup_max,
+                       "small_batch_cw": args.serve_small_batch, "backlog_cw": args.serve_backlog},
             "capacity_cw_s": round(cap, 1), "points": pts}
 
 
@@ -745,7 +749,7 @@ def workload_headline(args) -> dict:
         cap = serving_capacity(dec, args.seed + 1)
         r = serve_load(dec, cap, 0.7, args.serve_seconds, args.seed + 2, args.serve_slots,
                        args.serve_batch, args.serve_plain_below, args.serve_hold_above,
-                       args.serve_lanegroup_max)
+                       args.serve_lanegroup_max, args.serve_small_batch, args.serve_backlog)
         log(f"serving at 70 % of {cap:.0f} cw/s: {r}")
         out["latency"] = {"p50_ms": r["p50_ms"], "p99_ms": r["p99_ms"],
                           "workload": "serving loop, Poisson frames of U{1..256} codewords at "
@@ -779,6 +783,12 @@ def main() -> int:
     ap.add_argument("--serve-lanegroup-max", type=int, default=0,
                     help="a batch runs the lane-group kernel while it and the batches still "
                          "decoding hold at most this many codewords (0 = never)")
+    ap.add_argument("--serve-small-batch", type=int, default=None,
+                    help="batch size while the waiting queue is short (library default 256, "
+                         "0 = off)")
+    ap.add_argument("--serve-backlog", type=int, default=0,
+                    help="waiting codewords above which batches take everything that fits "
+                         "(0 = library default, 8 x small batch)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serve", action="store_true")

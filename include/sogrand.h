@@ -234,6 +234,12 @@ typedef struct sogrand_server_cfg {
                                 every batch still decoding hold at most this many
                                 codewords; FP64 final variant only (default 0 = never:
                                 at 70 % load its CU-sized workgroups lengthen the tail)    */
+    uint32_t small_batch_cw; /* while at most backlog_cw codewords wait, a batch takes at
+                                most this many (at least one request), so that a frame
+                                completes with its own slowest codeword rather than the
+                                slowest of a large batch (default 256; 0 = off)           */
+    uint32_t backlog_cw;     /* waiting codewords above which a batch takes everything that
+                                fits, for throughput (0 = 8 x small_batch_cw)             */
 } sogrand_server_cfg;
 
 typedef struct sogrand_completion {

@@ -95,6 +95,8 @@ class ServerCfg(C.Structure):
         ("plain_below_cw", C.c_uint32),
         ("hold_above_cw", C.c_uint32),
         ("lanegroup_max_cw", C.c_uint32),
+        ("small_batch_cw", C.c_uint32),
+        ("backlog_cw", C.c_uint32),
     ]
 
 
@@ -327,7 +329,8 @@ class Server:
 
     def __init__(self, decoder: Decoder, hbm_budget: int | None = None, max_inflight: int = 16,
                  max_batch_cw: int = 0, max_pending_cw: int = 1 << 20, with_soft: bool = False,
-                 plain_below_cw: int = 0, hold_above_cw: int = 0, lanegroup_max_cw: int = 0):
+                 plain_below_cw: int = 0, hold_above_cw: int = 0, lanegroup_max_cw: int = 0,
+                 small_batch_cw: int | None = None, backlog_cw: int = 0):
         lib = decoder._lib
         self._lib = lib
         cfg = ServerCfg()
@@ -341,6 +344,9 @@ class Server:
         cfg.plain_below_cw = plain_below_cw
         cfg.hold_above_cw = hold_above_cw
         cfg.lanegroup_max_cw = lanegroup_max_cw
+        if small_batch_cw is not None:
+            cfg.small_batch_cw = small_batch_cw
+        cfg.backlog_cw = backlog_cw
         h = C.c_void_p()
         _check(lib.sogrand_server_create(C.byref(h), decoder._h, C.byref(cfg)), "server_create")
         self._h = h

@@ -20,6 +20,7 @@
 // out of order. Here results are returned by request id.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cstdint>
 #include <condition_variable>
@@ -76,6 +77,8 @@ struct sogrand_server {
     size_t plain_below = 0; // in-flight codewords below which batches run the plain kernel
     size_t hold_above = 0;  // no new batch while this many codewords are in flight
     size_t lanegroup_max = 0; // batches up to this size run the lane-group kernel
+    size_t small_batch = 0;   // batch size while at most `backlog` codewords wait
+    size_t backlog = 0;
     size_t inflight_cw = 0; // codewords of the batches on the device (worker thread only)
     std::vector<Slot> slots;
     std::deque<Request> pending;
@@ -99,12 +102,18 @@ struct sogrand_server {
     void finish(Slot &s, bool event_ok);
 };
 
-// Move every waiting request that fits into slot s (caller holds the lock).
+// Move waiting requests into slot s (caller holds the lock): every one that
+// fits, or, while the queue is short, at most small_batch codewords (at least
+// one request).
 void sogrand_server::take(Slot &s)
 {
     s.reqs.clear();
     size_t n = 0;
-    while (!pending.empty() && n + pending.front().n <= batch_cw) {
+    // (never more than a slot holds)
+    const size_t limit =
+        small_batch && pending_cw <= backlog ? std::min(small_batch, batch_cw) : batch_cw;
+    while (!pending.empty() &&
+           (s.reqs.empty() ? pending.front().n <= batch_cw : n + pending.front().n <= limit)) {
         s.reqs.push_back(pending.front());
         n += pending.front().n;
         pending_cw -= pending.front().n;
@@ -124,6 +133,7 @@ void sogrand_server::launch(Slot &s)
         host_in = host_in || !r.llr_dev;
         n += r.n;
     }
+    if (n > batch_cw) ok = false; // more than the slot's buffers hold: fail the batch
     if (host_in && !s.h_llr &&
         hipHostMalloc((void **)&s.h_llr, batch_cw * sg::kCube * sizeof(double),
                       hipHostMallocDefault) != hipSuccess) {
@@ -302,6 +312,8 @@ void sogrand_server_cfg_default(sogrand_server_cfg *c)
     c->plain_below_cw = 0;
     c->hold_above_cw = 0;
     c->lanegroup_max_cw = 0;
+    c->small_batch_cw = 256;
+    c->backlog_cw = 0;
 }
 
 int sogrand_server_create(sogrand_server **out, sogrand_ctx *ctx, const sogrand_server_cfg *cfg)
@@ -341,6 +353,8 @@ int sogrand_server_create(sogrand_server **out, sogrand_ctx *ctx, const sogrand_
     // the lane-group kernel has the final decoder's FP64 arithmetic only
     const bool lg_ok = s->p.precision == SOGRAND_FP64 && s->p.variant == SOGRAND_VARIANT_FINAL;
     s->lanegroup_max = lg_ok ? (size_t)c.lanegroup_max_cw : 0;
+    s->small_batch = c.small_batch_cw;
+    s->backlog = c.backlog_cw ? (size_t)c.backlog_cw : (size_t)8 * c.small_batch_cw;
     if (hipSetDevice(ctx->device) != hipSuccess) {
         delete s;
         return SOGRAND_EHIP;
