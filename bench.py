@@ -275,11 +275,17 @@ def workload_serving(args) -> dict:
     import sogrand_amd as sg
 
     dec = sg.Decoder(0, backend=sg.BACKEND_PERSISTENT)
-    cap = serving_capacity(dec, args.seed)
+    if args.serve_capacity:
+        cap = float(args.serve_capacity)  # a fixed reference, for runs compared with each other
+    else:
+        cap = serving_capacity(dec, args.seed)
     log(f"serving capacity on the frame mix: {cap:.0f} cw/s")
+    loads = [float(x) for x in args.serve_loads.split(",")]
     pts = []
-    for load in (0.1, 0.3, 0.5, 0.7, 0.9, 1.0, 1.2):
-        r = serve_load(dec, cap, load, args.serve_seconds, args.seed + int(load * 100),
+    for k, load in enumerate(loads):
+        # a repeated load draws new frames (the seed advances per repeat)
+        r = serve_load(dec, cap, load, args.serve_seconds,
+                       args.seed + int(load * 100) + 1000 * loads[:k].count(load),
                        args.serve_slots, args.serve_batch, args.serve_plain_below,
                        args.serve_hold_above, args.serve_lanegroup_max, args.serve_small_batch,
                        args.serve_backlog)
@@ -783,6 +789,11 @@ def main() -> int:
     ap.add_argument("--serve-lanegroup-max", type=int, default=0,
                     help="a batch runs the lane-group kernel while it and the batches still "
                          "decoding hold at most this many codewords (0 = never)")
+    ap.add_argument("--serve-capacity", type=float, default=0.0,
+                    help="serving workload: offered loads relative to this rate in codewords/s "
+                         "(0 = measure the batched capacity first)")
+    ap.add_argument("--serve-loads", default="0.1,0.3,0.5,0.7,0.9,1.0,1.2",
+                    help="serving workload: the offered loads, as fractions of the capacity")
     ap.add_argument("--serve-small-batch", type=int, default=None,
                     help="batch size while the waiting queue is short (library default 256, "
                          "0 = off)")
