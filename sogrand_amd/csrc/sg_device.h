@@ -674,6 +674,31 @@ __device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c)
     return r;
 }
 
+// Whether any lane's chunk of eight position words has a pattern with bits
+// past the sixth (the words' top bit): three three-input ORs, one OR and one
+// ballot per chunk instead of an OR and a ballot per pair.
+__device__ __forceinline__ bool chunk_has_more(uint4 a, uint4 b)
+{
+    uint32_t o, p, q;
+    asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0xfe" : "=v"(o) : "v"(a.x), "v"(a.y), "v"(a.z));
+    asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0xfe" : "=v"(p) : "v"(a.w), "v"(b.x), "v"(b.y));
+    asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0xfe" : "=v"(q) : "v"(o), "v"(p), "v"(b.z));
+    return __ballot(((q | b.w) & kPosMore) != 0u) != 0ull;
+}
+
+// The four pairs of one chunk (position words a, b at schedule positions t ..
+// t + 7), the second of each pair dropped once the search stops.
+template <bool FINITE_T, class P, bool kMore, class LDS>
+__device__ __forceinline__ void search_chunk(SearchState &ss, LineList &ls, const LDS &S,
+                                             uint32_t s0, double pm_hd, int32_t ic,
+                                             const DecodeCfg &cfg, uint4 a, uint4 b, uint32_t t)
+{
+    search_pair<FINITE_T, P, kMore>(ss, ls, S, s0, pm_hd, ic, cfg, a.x, a.y, t);
+    if (!ss.done) search_pair<FINITE_T, P, kMore>(ss, ls, S, s0, pm_hd, ic, cfg, a.z, a.w, t + 2u);
+    if (!ss.done) search_pair<FINITE_T, P, kMore>(ss, ls, S, s0, pm_hd, ic, cfg, b.x, b.y, t + 4u);
+    if (!ss.done) search_pair<FINITE_T, P, kMore>(ss, ls, S, s0, pm_hd, ic, cfg, b.z, b.w, t + 6u);
+}
+
 // Field I of a position word (v_bfe_u32, written as the instruction: as a C
 // expression the compiler folds the extract into the address scaling, one
 // instruction more per field).
@@ -743,7 +768,7 @@ __device__ __forceinline__ void pattern_sums(const LDS &S, uint32_t s0, double p
 // (whose weight is the P_neg a budget stop leaves, cubic_decoder.c:638).
 // Without a budget the loop forms syndromes alone and a hit's metric is summed
 // from its mask afterwards, with the same additions.
-template <bool FINITE_T, class P, class LDS>
+template <bool FINITE_T, class P, bool kMore = true, class LDS>
 __device__ __forceinline__ void search_pair(SearchState &ss, LineList &ls, const LDS &S,
                                             uint32_t s0, double pm_hd, int32_t ic,
                                             const DecodeCfg &cfg, uint32_t w1, uint32_t w2,
@@ -767,7 +792,9 @@ __device__ __forceinline__ void search_pair(SearchState &ss, LineList &ls, const
     pattern_sums<P, kPmLoop>(S, s0, pm_hd, w1, w2, pm1, pm2, syn1, syn2);
     const int tbl = ic < kSchedGenericIc ? ic : kSchedGenericIc;
     const uint16_t *masks = cfg.sched + (size_t)tbl * kSchedStride;
-    if (__ballot(((w1 | w2) & kPosMore) != 0u) != 0ull) {
+    // kMore = false: no lane's chunk has a pattern with bits past the sixth
+    // (search_run tests that once per chunk)
+    if (kMore && __ballot(((w1 | w2) & kPosMore) != 0u) != 0ull) {
         auto more = [&](uint32_t m, Rec &pm, uint32_t &syn) {
 #pragma unroll 1
             for (uint32_t i = 0; i < kPosFields; ++i) m &= m - 1u;
@@ -851,10 +878,13 @@ __device__ __forceinline__ void search_run(SearchState &ss, LineList &ls, const 
             const uint4 *np = ss.tp + (t >> 2) + 2; // tables carry one chunk of padding
             b0 = np[0];
             b1 = np[1];
-            search_pair<FINITE_T, P>(ss, ls, S, s0, pm_hd, ic, cfg, ss.nxt.x, ss.nxt.y, t);
-            if (!ss.done) search_pair<FINITE_T, P>(ss, ls, S, s0, pm_hd, ic, cfg, ss.nxt.z, ss.nxt.w, t + 2u);
-            if (!ss.done) search_pair<FINITE_T, P>(ss, ls, S, s0, pm_hd, ic, cfg, ss.nxt2.x, ss.nxt2.y, t + 4u);
-            if (!ss.done) search_pair<FINITE_T, P>(ss, ls, S, s0, pm_hd, ic, cfg, ss.nxt2.z, ss.nxt2.w, t + 6u);
+            // (a wave-uniform branch: the rare chunks with long patterns take
+            // the variant that adds bits past the sixth; the single-precision
+            // variant, held to fewer registers, tests each pair)
+            if (P::kF32 || chunk_has_more(ss.nxt, ss.nxt2))
+                search_chunk<FINITE_T, P, true>(ss, ls, S, s0, pm_hd, ic, cfg, ss.nxt, ss.nxt2, t);
+            else
+                search_chunk<FINITE_T, P, false>(ss, ls, S, s0, pm_hd, ic, cfg, ss.nxt, ss.nxt2, t);
         }
         if (__ballot(!ss.done) == 0ull) break;
         if (!ss.done) {
@@ -862,10 +892,10 @@ __device__ __forceinline__ void search_run(SearchState &ss, LineList &ls, const 
             const uint4 *np = ss.tp + (t >> 2) + 2;
             ss.nxt = np[0];
             ss.nxt2 = np[1];
-            search_pair<FINITE_T, P>(ss, ls, S, s0, pm_hd, ic, cfg, b0.x, b0.y, t);
-            if (!ss.done) search_pair<FINITE_T, P>(ss, ls, S, s0, pm_hd, ic, cfg, b0.z, b0.w, t + 2u);
-            if (!ss.done) search_pair<FINITE_T, P>(ss, ls, S, s0, pm_hd, ic, cfg, b1.x, b1.y, t + 4u);
-            if (!ss.done) search_pair<FINITE_T, P>(ss, ls, S, s0, pm_hd, ic, cfg, b1.z, b1.w, t + 6u);
+            if (P::kF32 || chunk_has_more(b0, b1))
+                search_chunk<FINITE_T, P, true>(ss, ls, S, s0, pm_hd, ic, cfg, b0, b1, t);
+            else
+                search_chunk<FINITE_T, P, false>(ss, ls, S, s0, pm_hd, ic, cfg, b0, b1, t);
         }
     }
     if constexpr (!P::kF32 && !P::kLegacy) {
