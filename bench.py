@@ -806,7 +806,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the decoder shards codewords across GPUs with no "
-              f"exchange step, and this build runs on a single MI355X (asked for {args.gpus}).",
+              f"exchange step, and the bench runs on a single MI355X (asked for {args.gpus}).",
               file=sys.stderr)
         return 2
     args.n_cw_set = args.n_cw is not None

@@ -1,4 +1,4 @@
-"""bench.py's command-line contract that needs no GPU: this build measures one
+"""bench.py's command-line contract that needs no GPU: the bench measures one
 MI355X, so a multi-GPU request exits with a message instead of a number."""
 import os
 import subprocess
