@@ -440,6 +440,37 @@ def test_cli_legacy_variant(tmp_path):
     assert lines[5] == f"Average NG_p per block: {g['NG_p_3.0'].sum() / n:.2f}"
 
 
+def test_cli_options_match_library(tmp_path):
+    # every kernel and the FP32 variant through the drop-in's flags: bits,
+    # statistics and soft outputs equal the library's for the same settings;
+    # a setting the kernel does not support exits 1 with the library's message
+    sg = _sg()
+    n = 20
+    llr = sg.gen_llr(33, n, ebn0_db=np.linspace(1.0, 3.0, n))
+    fin = tmp_path / "llr.bin"
+    llr.astype("<f8").tofile(fin)
+    cases = [(["--backend", "plain"], dict(backend=0)),
+             (["--backend", "persistent"], dict(backend=1)),
+             (["--backend", "lanegroup"], dict(backend=2)),
+             (["--precision", "fp32"], dict(backend=1, precision=1)),
+             (["--backend", "lanegroup", "--tmax", "300"], dict(backend=2, Tmax=300))]
+    for flags, params in cases:
+        _say("cli", *flags)
+        fout, fst, fsoft = tmp_path / "dec.bin", tmp_path / "st.bin", tmp_path / "soft.bin"
+        r = subprocess.run([sg.CLI_PATH, str(fin), str(fout), "--stats", str(fst),
+                            "--soft-out", str(fsoft), *flags],
+                           capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, (flags, r.stderr)
+        with sg.Decoder(0, **params) as dec:  # the CLI's defaults otherwise
+            ref = dec.decode(llr, soft=True)
+        assert np.array_equal(np.fromfile(fout, np.uint8).reshape(n, 64), ref.bits), flags
+        assert np.fromfile(fst, sg.CW_STATS_DTYPE).tolist() == ref.stats.tolist(), flags
+        assert np.array_equal(np.fromfile(fsoft, "<f8").reshape(n, 4096), ref.soft), flags
+    r = subprocess.run([sg.CLI_PATH, str(fin), str(tmp_path / "x.bin"), "--backend", "lanegroup",
+                        "--precision", "fp32"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 1 and r.stderr.startswith("Error: "), r.stderr
+
+
 @pytest.mark.parametrize("backend", [0, 1, 2], ids=["plain", "persistent", "lanegroup"])
 def test_graph_replays_match_eager(decoder, backend):
     # a captured launch sequence replayed several times, with eager decodes on
