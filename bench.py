@@ -93,11 +93,12 @@ def cpu_baseline(seed: int, ebn0: float, seconds: float) -> dict:
 
     cores = host_cores()
     o = Oracle()
-    # size the sample from a short single-codeword probe on all cores
-    probe = sg.gen_llr(seed + 7, cores, ebn0_db=ebn0)
+    # size the sample from a short probe of eight codewords per core (with one,
+    # the slowest codeword sets the probe's time and the sample comes out short)
+    probe = sg.gen_llr(seed + 7, 8 * cores, ebn0_db=ebn0)
     t0 = time.perf_counter()
     o.decode(probe, threads=cores)
-    rate = cores / max(time.perf_counter() - t0, 1e-3)
+    rate = 8 * cores / max(time.perf_counter() - t0, 1e-3)
     n = max(cores, int(seconds * rate))
     llr = sg.gen_llr(seed, n, ebn0_db=ebn0)
     t0 = time.perf_counter()
