@@ -590,6 +590,7 @@ def workload_latency(args) -> dict:
                 elif not ((got[0] == ref[0]).all() and (got[1] == ref[1]).all()):
                     raise RuntimeError(f"backend {name} differs at {n} codewords, {ebn0} dB")
                 row[name + "_ms"] = round(best, 3)
+            row["NG_total"] = int(ref[1]["NG"].sum())  # guesses of the frame, every backend
             log(f"{ebn0} dB, {n} cw: {row}")
             rows.append(row)
     return {"metric": "frame decode time by backend (kernel, HIP events)", "unit": "ms",

@@ -57,6 +57,9 @@ extern "C" {
 /* backend */
 #define SOGRAND_BACKEND_PLAIN 0      /* one workgroup per codeword, static lines */
 #define SOGRAND_BACKEND_PERSISTENT 1 /* persistent waves + device queue          */
+#define SOGRAND_BACKEND_AUTO 3       /* by batch size: plain up to one cube per resident
+                                        persistent wave, then persistent with one cube
+                                        slot per wave, two from twice that on */
 #define SOGRAND_BACKEND_LANEGROUP 2  /* one 1024-lane workgroup per codeword, the
                                         cube in LDS, four lanes per line: the
                                         shortest time per codeword (small frames);

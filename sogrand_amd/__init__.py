@@ -38,6 +38,7 @@ VARIANT_LEGACY = 1  # SOGRAND_C/cubic_decoder.c, the earlier decoder (FP64 only)
 BACKEND_PLAIN = 0
 BACKEND_PERSISTENT = 1
 BACKEND_LANEGROUP = 2  # cube in LDS, four lanes per line (FP64, final variant)
+BACKEND_AUTO = 3  # plain or persistent (one or two cube slots per wave) by batch size
 UINT64_MAX = (1 << 64) - 1
 
 CW_STATS_DTYPE = np.dtype(

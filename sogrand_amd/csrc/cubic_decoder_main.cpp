@@ -7,8 +7,9 @@
 // so it slots into cubic_flow.sh unchanged. Options (after the two paths):
 //   --tmax N            guess budget per component decode (default: unlimited)
 //   --precision P       fp64 (default, bit-exact with SOGRAND_C) | fp32
-//   --backend B         persistent (default) | plain | lanegroup (fp64, final variant:
-//                       the shortest time for files of up to one codeword per CU)
+//   --backend B         auto (default: plain, or persistent by batch size) | persistent |
+//                       plain | lanegroup (fp64, final variant: the shortest time for
+//                       files of up to one codeword per CU)
 //   --epsilon E         EPSILON of the soft output, e.g. 2.220446049250313e-16 for the
 //                       reference's cubic_decoder_updated.c (default 1e-40)
 //   --variant V         final (default: cubic_decoder_final.c) | legacy (the earlier
@@ -44,6 +45,7 @@ int main(int argc, char *argv[])
     const char *out_path = argv[2];
     sogrand_params p;
     sogrand_params_default(&p);
+    p.backend = SOGRAND_BACKEND_AUTO; // files of any size: the kernel that suits it
     const char *stats_path = nullptr, *soft_path = nullptr;
     int device = 0, dim = 3;
     for (int i = 3; i < argc; ++i) {
@@ -67,6 +69,7 @@ int main(int argc, char *argv[])
             if (v == "plain") p.backend = SOGRAND_BACKEND_PLAIN;
             else if (v == "persistent") p.backend = SOGRAND_BACKEND_PERSISTENT;
             else if (v == "lanegroup") p.backend = SOGRAND_BACKEND_LANEGROUP;
+            else if (v == "auto") p.backend = SOGRAND_BACKEND_AUTO;
             else return usage(argv[0]);
         } else if (a == "--epsilon") {
             p.epsilon = std::strtod(need(), nullptr);

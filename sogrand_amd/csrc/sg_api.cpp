@@ -59,13 +59,28 @@ LaunchPlan plan_launch(const sogrand_ctx *ctx, const sogrand_params &p, size_t n
     LaunchPlan lp{};
     lp.persistent = p.backend == SOGRAND_BACKEND_PERSISTENT;
     lp.lanegroup = p.backend == SOGRAND_BACKEND_LANEGROUP;
+    int auto_slots = 2;
+    if (p.backend == SOGRAND_BACKEND_AUTO) {
+        // W = the persistent waves resident at two cube slots each (3,072 on
+        // MI355X). Up to W cubes a persistent wave would hold one cube with 64
+        // lanes where the plain kernel gives it 256, and its queue has nothing
+        // to balance: plain (6.7 against 27.3 ms for 256 codewords at 2 dB).
+        // Up to 2W, one slot per wave fills the device sooner than two
+        // (4,096 codewords: 37.3 against 38.9 ms at 2 dB, 171 against 229 at
+        // 1 dB); from there on two slots (16,384: 94.9 against 104.2 ms).
+        const uint64_t tmax = effective_tmax(p.Tmax);
+        const size_t w = (size_t)ctx->cu_count * kPersistentWaves *
+                         persistent_blocks_per_cu(2, tmax_binds(tmax), arith_of(p));
+        lp.persistent = n > w;
+        auto_slots = n <= 2 * w ? 1 : 2;
+    }
     if (lp.lanegroup) {
         // the cube lives in LDS: no L_E scratch
         lp.blocks = (int)lanegroup_blocks(n, ctx->cu_count);
         lp.slots = 1;
         lp.le_slots = 0;
     } else if (lp.persistent) {
-        const int slots = p.wave_slots > 0 ? p.wave_slots : 2;
+        const int slots = p.wave_slots > 0 ? p.wave_slots : auto_slots;
         const int wpb = kPersistentWaves;
         // a persistent grid must be resident all at once: never more workgroups
         // per CU than the occupancy the kernel's registers and LDS allow
@@ -171,7 +186,7 @@ static int check_params(const sogrand_params *p)
     if (!std::isfinite(p->alpha)) return SOGRAND_EINVAL;
     if (p->precision != SOGRAND_FP64 && p->precision != SOGRAND_FP32_MIXED) return SOGRAND_EINVAL;
     if (p->backend != SOGRAND_BACKEND_PLAIN && p->backend != SOGRAND_BACKEND_PERSISTENT &&
-        p->backend != SOGRAND_BACKEND_LANEGROUP)
+        p->backend != SOGRAND_BACKEND_LANEGROUP && p->backend != SOGRAND_BACKEND_AUTO)
         return SOGRAND_EINVAL;
     // the lane-group kernel has the final decoder's double-precision arithmetic only
     if (p->backend == SOGRAND_BACKEND_LANEGROUP &&

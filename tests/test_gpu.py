@@ -453,7 +453,8 @@ def test_cli_options_match_library(tmp_path):
              (["--backend", "persistent"], dict(backend=1)),
              (["--backend", "lanegroup"], dict(backend=2)),
              (["--precision", "fp32"], dict(backend=1, precision=1)),
-             (["--backend", "lanegroup", "--tmax", "300"], dict(backend=2, Tmax=300))]
+             (["--backend", "lanegroup", "--tmax", "300"], dict(backend=2, Tmax=300)),
+             ([], dict(backend=3))]
     for flags, params in cases:
         _say("cli", *flags)
         fout, fst, fsoft = tmp_path / "dec.bin", tmp_path / "st.bin", tmp_path / "soft.bin"
@@ -469,6 +470,29 @@ def test_cli_options_match_library(tmp_path):
     r = subprocess.run([sg.CLI_PATH, str(fin), str(tmp_path / "x.bin"), "--backend", "lanegroup",
                         "--precision", "fp32"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 1 and r.stderr.startswith("Error: "), r.stderr
+
+
+def test_auto_backend_matches_explicit_kernels(decoder):
+    # the auto backend picks plain below one cube per resident persistent wave
+    # (3,072 on MI355X), persistent with one slot per wave up to twice that and
+    # two slots above: every pick decodes alike
+    sg = _sg()
+    from sogrand_amd import hip
+
+    for n in (300, 4000, 7000):
+        _say("auto", n)
+        llr = sg.gen_llr(515 + n, n, ebn0_db=np.linspace(1.5, 3.0, n))
+        d = hip.DeviceArray.from_host(llr)
+        bits = hip.DeviceArray((n, 64), np.uint8)
+        st = hip.DeviceArray((n,), sg.CW_STATS_DTYPE)
+        out = []
+        for backend in (3, 0, 1):
+            decoder.set_params(backend=backend)
+            decoder.decode(d, out_bits=bits, out_stats=st)
+            out.append((bits.to_host().copy(), st.to_host().copy()))
+        for b, s in out[1:]:
+            assert (b == out[0][0]).all() and (s == out[0][1]).all(), n
+    decoder.set_params(backend=1)
 
 
 @pytest.mark.parametrize("backend", [0, 1, 2], ids=["plain", "persistent", "lanegroup"])

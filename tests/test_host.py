@@ -132,9 +132,13 @@ def test_lanegroup_backend_parameter_checks():
             sg.Decoder(0, backend=sg.BACKEND_LANEGROUP, **kw)
         assert e.value.code == sg.EINVAL, kw
     with pytest.raises(sg.SograndError) as e:
-        sg.Decoder(0, backend=3)
+        sg.Decoder(0, backend=4)
     assert e.value.code == sg.EINVAL
     for kw in ({}, {"epsilon": 2.220446049250313e-16}, {"Tmax": 1000}, {"L": 1}):
         with pytest.raises(sg.SograndError) as e:
             sg.Decoder(0, backend=sg.BACKEND_LANEGROUP, **kw)
         assert e.value.code == sg.ENODEV, kw  # past the checks, then no device
+    for kw in ({}, {"precision": sg.FP32_MIXED}, {"variant": sg.VARIANT_LEGACY}):
+        with pytest.raises(sg.SograndError) as e:
+            sg.Decoder(0, backend=sg.BACKEND_AUTO, **kw)
+        assert e.value.code == sg.ENODEV, kw
